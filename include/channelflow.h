@@ -1,0 +1,142 @@
+/*
+ * channelflow.h -- C-ABI of the MI355X-native RK3 timestep of the
+ * pseudo-spectral channel-flow LES (reference: fxie2/paes-channelflow, Fortran 90).
+ *
+ * One context owns one GPU, one HIP stream and every device buffer of the run
+ * (fields, RK registers, pressure, 3/2-grid intermediates, FFT twiddles).  The
+ * host never holds device pointers: fields cross the boundary as plain
+ * column-major REAL(8) host arrays (x fastest, then y, then z), exactly the
+ * memory order of the reference's u(nnx,nny,iz) planes (modules.f90:21-26).
+ *
+ * The reference has no C interface; its hot path is a set of Fortran
+ * subroutines with implicit interfaces that read and write module globals.
+ * Each entry point below names the reference code it replaces.  The Fortran
+ * driver (paes-channelflow_amd/fortran/les_gpu.F90) binds them with
+ * ISO_C_BINDING.
+ *
+ * Return codes: 0 = success, < 0 = error (message via cf_last_error).
+ * Threading: a context is used by one host thread at a time.  Every call is
+ * synchronous on return except cf_stage/cf_step, which enqueue work on the
+ * context's stream; errors raised on the device (the utau > 10 guard, a
+ * non-finite field) are reported by the next cf_sync / cf_get_* call.
+ */
+#ifndef CHANNELFLOW_H
+#define CHANNELFLOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_OK 0
+#define CF_E_ARG (-1)         /* invalid argument or unsupported option          */
+#define CF_E_HIP (-2)         /* HIP runtime error (allocation, launch, copy)     */
+#define CF_E_UTAU (-3)        /* utau > 10 wall-model guard: lowerbc.f90:77-80,    */
+                              /* upperbc.f90:253-256 (the reference calls lesabort) */
+#define CF_E_NONFINITE (-4)   /* a field became NaN/Inf                           */
+#define CF_E_ECLAMP (-5)      /* the 3/2-grid SGS TKE fell below emin on some      */
+                              /* level; that clamp (comp1.f90:109,121) is not      */
+                              /* supported on the device path                      */
+#define CF_E_NODEV (-6)       /* no HIP device                                     */
+
+#define CF_PREC_FP64 0        /* every field and transform in FP64                  */
+#define CF_PREC_FP32 1        /* FP32 fields/FFTs/products/SGS; FP64 pressure RHS,  */
+                              /* tridiagonal solve, projection and update           */
+
+typedef struct cf_ctx cf_ctx;
+
+/* Run parameters: the namelist groups GRIDSIZE/PARAMS/TIMING of les.F90:30-34
+ * plus the precision mode.  Derived constants follow les.F90:92-121. */
+typedef struct cf_params {
+  int32_t nnx, nny, nnz;   /* grid; nnx == nny in {32,64,128,256,512}, nnz >= 4 */
+  double xl, yl, zl;       /* domain lengths                                    */
+  double z0;               /* roughness length (wall model)                     */
+  double dt;               /* fixed time step (xkmax = dz^2/dt/5, les.F90:216)   */
+  int32_t imodel;          /* SGS length scale; 3 supported (complen.f90:88-89) */
+  int32_t isurface;        /* wall model; 1 supported (lowerbc.f90:128-130)     */
+  int32_t precision;       /* CF_PREC_FP64 or CF_PREC_FP32                       */
+  int32_t device;          /* HIP device ordinal                                 */
+  double alk;              /* SGS length of imodel 3; <= 0 selects dsl.  The    */
+                           /* reference never assigns it (allocate.f90:61).     */
+  double ugal, vgal;       /* Galilean shifts of the wall model (storage module) */
+} cf_params;
+
+/* Set-up: replaces setcombuf/allocate_arrays/initwave/fft_init
+ * (les.F90:146-149; buffers.f90:1-34, allocate.f90:1-127, init.f90:4-44,
+ * filter_fftw.f90:16-75).  Allocates every device buffer once. */
+int cf_create(const cf_params* p, cf_ctx** out);
+
+/* Teardown: fft_finalize + deallocate_arrays (les.F90:356-357). */
+int cf_destroy(cf_ctx* ctx);
+
+/* Load the interior levels 1..nnz of u, v, w, e (each nnx*nny*nnz doubles;
+ * w at faces 1..nnz, w(nnz) is the top wall) and apply deal() to all four
+ * (les.F90:185-190, filter_fftw.f90:316-329).  Replaces read_field/initvel
+ * (les.F90:158-173).  RK registers and pressure are zeroed. */
+int cf_set_state(cf_ctx* ctx, const double* u, const double* v, const double* w,
+                 const double* e);
+
+/* Read the interior levels 1..nnz of u, v, w, e and p (any pointer may be
+ * NULL).  Replaces the per-level record of sav_field (read_write.f90:117-126). */
+int cf_get_state(cf_ctx* ctx, double* u, double* v, double* w, double* e, double* p);
+
+/* Read the RK registers ru, rv, rw, re (levels 1..nnz, pressure gradient
+ * included: the values comp2 leaves behind, comp2.f90:238-254).  Any pointer
+ * may be NULL. */
+int cf_get_tendency(cf_ctx* ctx, double* ru, double* rv, double* rw, double* re);
+
+/* Read the comp1 tendencies of the last stage before the pressure projection
+ * (ru, rv, rw after compress and the rw mean removal, comp1.f90:161-177). */
+int cf_get_comp1_tendency(cf_ctx* ctx, double* ru, double* rv, double* rw);
+
+/* One RK3 stage istage in {1,2,3} with the fixed dt: the body of the stage
+ * loop les.F90:254-285 -- lower, upper, comp1, comprhs, press, comp2.  When
+ * mnout != 0 and istage == 1, the SGS stress means of compsgs3.f90:236-243 are
+ * accumulated.  Enqueued on the context stream. */
+int cf_stage(cf_ctx* ctx, int istage, int mnout);
+
+/* nsteps full RK3 steps (3 stages each) without means output.  Enqueued. */
+int cf_step(cf_ctx* ctx, int nsteps);
+
+/* Change dt (and xkmax) for subsequent stages. */
+int cf_set_dt(cf_ctx* ctx, double dt);
+
+/* Wait for the stream and report any device-side error flag. */
+int cf_sync(cf_ctx* ctx);
+
+/* Mean-square 3/2-grid divergence of the last stage's input field per level
+ * (divz, comp1.f90:145), nnz doubles. */
+int cf_get_divz(cf_ctx* ctx, double* divz);
+
+/* utau, utau_s, dudt, dvdt as left by sufsim/sufsim_upper (lowerbc.f90:67-72,
+ * upperbc.f90:251): 4 doubles. */
+int cf_get_scalars(cf_ctx* ctx, double* scalars);
+
+/* Mean profiles (compmn.F90:1-64) into meanbuf[19*nnz] in the row order of
+ * allocate.f90:272-294 (englez, engsbz, divz, uxym, vxym, wxym, uule, vvle,
+ * wwle, uwle, vwle, uvle, pple, uusb, vvsb, uvsb, wwsb, uwsb, vwsb), stored
+ * meanbuf[row + 19*(iz-1)] like the Fortran meanbuf(19,nnz). */
+int cf_compmn(cf_ctx* ctx, double* meanbuf);
+
+/* Time nsteps steps with HIP events on the context stream.  Writes the total
+ * milliseconds and, when kernel_ms is not NULL, the average duration of each
+ * kernel family of one step measured with events bracketing every launch
+ * (kernel_ms[CF_NKERN]; the per-kernel pass runs after the total timing). */
+#define CF_NKERN 8
+int cf_time_steps(cf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms);
+
+/* Name of kernel family k of cf_time_steps (0 <= k < CF_NKERN). */
+const char* cf_kernel_name(int k);
+
+/* Bytes of device memory the context holds. */
+int64_t cf_device_bytes(const cf_ctx* ctx);
+
+/* Text of the last error of this context (or of cf_create when ctx is NULL). */
+const char* cf_last_error(const cf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHANNELFLOW_H */

@@ -1,0 +1,237 @@
+"""Python host binding of the MI355X channel-flow RK3 timestep.
+
+This is a thin ctypes layer over the C-ABI declared in ``include/channelflow.h``
+(the same entry points the Fortran driver binds with ISO_C_BINDING).  It
+mirrors the reference driver's vocabulary: a run is created from the
+GRIDSIZE/PARAMS/TIMING namelist values (les.F90:30-34), fields are loaded like
+a restart file (read_field, read_write.f90:1-11), advanced stage by stage
+(lower/upper/comp1/comprhs/press/comp2, les.F90:254-285) and read back like
+sav_field (read_write.f90:117-126).
+
+The shared library must have been built (``__graft_entry__.build()``); if it
+is missing, importing this package raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libchannelflow.so")
+
+CF_OK = 0
+CF_E_ARG = -1
+CF_E_HIP = -2
+CF_E_UTAU = -3
+CF_E_NONFINITE = -4
+CF_E_ECLAMP = -5
+CF_E_NODEV = -6
+CF_PREC_FP64 = 0
+CF_PREC_FP32 = 1
+CF_NKERN = 8
+
+EXPORTS = (
+    "cf_create", "cf_destroy", "cf_set_state", "cf_get_state", "cf_get_tendency",
+    "cf_get_comp1_tendency", "cf_stage", "cf_step", "cf_set_dt", "cf_sync", "cf_get_divz",
+    "cf_get_scalars", "cf_compmn", "cf_time_steps", "cf_kernel_name", "cf_device_bytes",
+    "cf_last_error",
+)
+
+
+class CfParams(ctypes.Structure):
+    """struct cf_params of include/channelflow.h."""
+    _fields_ = [
+        ("nnx", ctypes.c_int32), ("nny", ctypes.c_int32), ("nnz", ctypes.c_int32),
+        ("xl", ctypes.c_double), ("yl", ctypes.c_double), ("zl", ctypes.c_double),
+        ("z0", ctypes.c_double), ("dt", ctypes.c_double),
+        ("imodel", ctypes.c_int32), ("isurface", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("alk", ctypes.c_double), ("ugal", ctypes.c_double), ("vgal", ctypes.c_double),
+    ]
+
+
+class ChannelFlowError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"channelflow error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libchannelflow.so and declare the C signatures.  Raises when the
+    library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    vp = ctypes.c_void_p
+    lib.cf_create.argtypes = [ctypes.POINTER(CfParams), ctypes.POINTER(vp)]
+    lib.cf_destroy.argtypes = [vp]
+    lib.cf_set_state.argtypes = [vp, dp, dp, dp, dp]
+    lib.cf_get_state.argtypes = [vp, dp, dp, dp, dp, dp]
+    lib.cf_get_tendency.argtypes = [vp, dp, dp, dp, dp]
+    lib.cf_get_comp1_tendency.argtypes = [vp, dp, dp, dp]
+    lib.cf_stage.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.cf_step.argtypes = [vp, ctypes.c_int]
+    lib.cf_set_dt.argtypes = [vp, ctypes.c_double]
+    lib.cf_sync.argtypes = [vp]
+    lib.cf_get_divz.argtypes = [vp, dp]
+    lib.cf_get_scalars.argtypes = [vp, dp]
+    lib.cf_compmn.argtypes = [vp, dp]
+    lib.cf_time_steps.argtypes = [vp, ctypes.c_int, dp, dp]
+    lib.cf_kernel_name.argtypes = [ctypes.c_int]
+    lib.cf_kernel_name.restype = ctypes.c_char_p
+    lib.cf_device_bytes.argtypes = [vp]
+    lib.cf_device_bytes.restype = ctypes.c_int64
+    lib.cf_last_error.argtypes = [vp]
+    lib.cf_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name not in ("cf_kernel_name", "cf_device_bytes", "cf_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def z0_of_retau(re_tau: float, h: float = 1.0, kappa: float = 0.4, B: float = 5.0) -> float:
+    """Roughness length standing in for Re_tau: the reference has no molecular
+    viscosity, so Re_tau enters only through z0 = h exp(-kappa B) / Re_tau."""
+    return h * math.exp(-kappa * B) / re_tau
+
+
+@dataclass
+class RunParams:
+    nnx: int
+    nny: int
+    nnz: int
+    xl: float = 2.0 * math.pi
+    yl: float = math.pi
+    zl: float = 2.0
+    z0: float = 7.519e-4
+    dt: float = 0.01
+    imodel: int = 3
+    isurface: int = 1
+    precision: int = CF_PREC_FP64
+    device: int = 0
+    alk: float = 0.0
+    ugal: float = 0.0
+    vgal: float = 0.0
+
+    def to_c(self) -> CfParams:
+        return CfParams(self.nnx, self.nny, self.nnz, self.xl, self.yl, self.zl, self.z0, self.dt,
+                        self.imodel, self.isurface, self.precision, self.device, self.alk,
+                        self.ugal, self.vgal)
+
+
+class ChannelFlow:
+    """One run on one GPU (a cf_ctx)."""
+
+    def __init__(self, params: RunParams):
+        self.lib = load_library()
+        self.params = params
+        self._h = ctypes.c_void_p()
+        cp = params.to_c()
+        rc = self.lib.cf_create(ctypes.byref(cp), ctypes.byref(self._h))
+        if rc != CF_OK:
+            raise ChannelFlowError(rc, self.lib.cf_last_error(None).decode())
+        self.shape = (params.nnz, params.nny, params.nnx)
+
+    def _check(self, rc):
+        if rc != CF_OK:
+            raise ChannelFlowError(rc, self.lib.cf_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self.lib.cf_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"expected shape {self.shape}, got {a.shape}")
+        return a
+
+    def set_state(self, u, v, w, e):
+        u, v, w, e = (self._arr(x) for x in (u, v, w, e))
+        self._check(self.lib.cf_set_state(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(e)))
+
+    def get_state(self):
+        out = [np.empty(self.shape) for _ in range(5)]
+        self._check(self.lib.cf_get_state(self._h, *(_ptr(a) for a in out)))
+        return dict(zip("uvwep", out))
+
+    def get_tendency(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        self._check(self.lib.cf_get_tendency(self._h, *(_ptr(a) for a in out)))
+        return dict(zip(("ru", "rv", "rw", "re"), out))
+
+    def get_comp1_tendency(self):
+        out = [np.empty(self.shape) for _ in range(3)]
+        self._check(self.lib.cf_get_comp1_tendency(self._h, *(_ptr(a) for a in out)))
+        return dict(zip(("ru", "rv", "rw"), out))
+
+    def stage(self, istage: int, mnout: bool = False):
+        self._check(self.lib.cf_stage(self._h, istage, int(bool(mnout))))
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.cf_step(self._h, nsteps))
+
+    def set_dt(self, dt: float):
+        self._check(self.lib.cf_set_dt(self._h, dt))
+
+    def sync(self):
+        self._check(self.lib.cf_sync(self._h))
+
+    def divz(self):
+        out = np.empty(self.params.nnz)
+        self._check(self.lib.cf_get_divz(self._h, _ptr(out)))
+        return out
+
+    def scalars(self):
+        out = np.empty(4)
+        self._check(self.lib.cf_get_scalars(self._h, _ptr(out)))
+        return dict(zip(("utau", "utau_s", "dudt", "dvdt"), out))
+
+    def compmn(self):
+        """meanbuf(19, nnz) in the allocate.f90:272-294 row order."""
+        out = np.empty((self.params.nnz, 19))
+        self._check(self.lib.cf_compmn(self._h, _ptr(out)))
+        return out.T.copy()
+
+    def time_steps(self, nsteps: int, per_kernel: bool = False):
+        tot = np.zeros(1)
+        ker = np.zeros(CF_NKERN) if per_kernel else None
+        self._check(self.lib.cf_time_steps(self._h, nsteps, _ptr(tot), _ptr(ker)))
+        if per_kernel:
+            names = [self.lib.cf_kernel_name(k).decode() for k in range(CF_NKERN)]
+            return float(tot[0]), dict(zip(names, ker))
+        return float(tot[0])
+
+    def device_bytes(self) -> int:
+        return int(self.lib.cf_device_bytes(self._h))

@@ -1,0 +1,931 @@
+// cf_kernels.hpp -- HIP kernels of one RK3 stage of the channel-flow LES.
+//
+// Data layout in HBM (level L of a field lives in slot L, slots 0..nnz+1):
+//   band spectrum  S[L][j][kx]  complex, j in [0,ny) (ky = j, or j-ny above
+//                  ny/2; row ny/2 is the Nyquist row and stays zero),
+//                  kx in [0, nx/2)  -- the band kept by expand/compress
+//                  (filter_fftw.f90:269-272,300-303).
+//   x-spectral     X[L][y][kx]  complex, y a physical row of the small (ny) or
+//                  3/2 (nyb) grid, kx in [0, nx/2).
+//   physical       P[L][y][x]   real, the reference's u(nnx,nny,iz) order.
+//
+// Stage pipeline (the reference's comp1 -> comprhs -> press -> comp2):
+//   k_wall       plane means of level 1 / nnz -> utau, wall-stress factors
+//                (sufsim, lowerbc.f90:57-91; sufsim_upper, upperbc.f90:241-267)
+//   k_col_synth  y-inverse of u,v,w,e and d/dy of each onto the 3/2 grid
+//                (expand + yderiv, filter_fftw.f90:194-278)
+//   k_rows       per 3/2-grid row and z-chunk: x-inverse of the 12 fields
+//                needed at a level (expand + xderiv), the whole pointwise
+//                tendency of comp1 (compadv1, complen, compsgs1, compsgs3,
+//                wall stress, forcing), and the x-forward of the fluxes,
+//                i kx F_x + H already combined (xderiv + compress)
+//   k_col_anal   y-forward of the fluxes, i ky F_y added, truncation to the
+//                band (compress) and rw mean removal (comp1.f90:161-177)
+//   k_press      one (kx,ky) mode per lane, sweep in z: RK substep
+//                (comp1.f90:126-133), Poisson RHS (comprhs.f90), Thomas
+//                solve (press.F90, tridv.f90) in FP64 and, on the way back
+//                up, the projection and RK update (comp2.f90)
+//   k_erows      the SGS TKE update in physical space (comp2.f90:261) and
+//                its x-forward for the next stage
+#pragma once
+#include "cf_fft.hpp"
+
+namespace cf {
+
+// ---------------------------------------------------------------------------
+// Compile-time shape helpers
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int NY>
+struct ColShape {
+  static constexpr int MX = NX / 2;
+  static constexpr int NYB = 3 * NY / 2;
+  static constexpr int BYTES = 4 * NY * (int)sizeof(cx<T>);  // per column of the tile
+  static constexpr int TK0 = 65536 / BYTES;
+  static constexpr int TK = TK0 < MX ? TK0 : MX;             // columns per workgroup
+  static constexpr int NT = 256;
+};
+
+template <typename T, int NX>
+struct RowShape {
+  static constexpr int MX = NX / 2;
+  static constexpr int NXB = 3 * NX / 2;
+  static constexpr int NT = MX;          // threads; each owns 3 points of a 3/2-grid row
+  static constexpr int PPT = 3;
+  static constexpr int NB = (6 * NXB * (int)sizeof(cx<T>) <= 65536) ? 6 : 3;  // transforms per batch
+};
+
+// Device-side scalars shared by the stage kernels.
+struct WallScalars {
+  double c_lo, c_hi;          // utau^2/vsfc of the lower and upper wall
+  double utau_lo, utau_hi;
+  double utau, utau_s, dudt, dvdt;
+  int flags;                  // bit 0: utau > 10, bit 1: e clamp, bit 2: non-finite
+  int pad;
+};
+
+enum : int { FLAG_UTAU = 1, FLAG_ECLAMP = 2, FLAG_NONFINITE = 4 };
+
+struct Consts {
+  double dzm, dzm2, todzm, dz;
+  double alk, xkmax, ceps, emin;
+  double ugal, vgal;
+  double dtab1, dtab2, dtabm;
+  double xymb, zody, vk, zl;
+  int nz, istage, mnout, zchunk;
+};
+
+// ---------------------------------------------------------------------------
+// k_wall: wall-model scalars (one thread)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_wall(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
+                       const cx<T>* __restrict__ Ruo, const cx<T>* __restrict__ Rvo,
+                       size_t plane, Consts c, WallScalars* ws) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nz = c.nz;
+  // plane means = the (0,0) coefficient (lowerbc.f90:57-58)
+  double u1 = (double)U[1 * plane].x, v1 = (double)V[1 * plane].x;
+  double vs = sqrt(u1 * u1 + v1 * v1);
+  vs = vs > 0.01 ? vs : 0.01;                                   // lowerbc.f90:65
+  double ut = vs * c.vk / c.zody;                               // lowerbc.f90:67
+  ws->utau_lo = ut;
+  ws->c_lo = ut * ut / vs;
+  double un = (double)U[(size_t)nz * plane].x + c.ugal;        // upperbc.f90:241-242
+  double vn = (double)V[(size_t)nz * plane].x + c.vgal;
+  double vs2 = sqrt(un * un + vn * vn);
+  vs2 = vs2 > 0.01 ? vs2 : 0.01;
+  double ut2 = vs2 * c.vk / c.zody;
+  ws->utau_hi = ut2;
+  ws->c_hi = ut2 * ut2 / vs2;
+  ws->utau = ut2;                                               // sufsim_upper runs last
+  double su = 0.0, sv = 0.0;                                    // lowerbc.f90:70-72
+  for (int z = 1; z <= nz; ++z) {
+    su += (double)Ruo[(size_t)z * plane].x;
+    sv += (double)Rvo[(size_t)z * plane].x;
+  }
+  ws->dudt = su / nz;
+  ws->dvdt = sv / nz;
+  ws->utau_s = sqrt(c.zl / 2.0 * sqrt(ws->dvdt * ws->dvdt + (0.016 - ws->dudt) * (0.016 - ws->dudt)));
+  if (!(ut <= 10.0) || !(ut2 <= 10.0)) ws->flags |= FLAG_UTAU;  // lowerbc.f90:77-80
+}
+
+// ---------------------------------------------------------------------------
+// k_col_synth: band spectrum -> 3/2-grid x-spectral rows, value and d/dy
+// grid (MX/TK, nz, 4 fields u,v,w,e)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SynthArgs {
+  const cx<T>* S[3];      // U, V, W band spectra
+  const cx<T>* Ex;        // e, x-spectral small grid (rows of physical y)
+  cx<T>* B;               // 8 outputs: u, uy, v, vy, w, wy, e, ey ; stride bstride
+  size_t bstride;         // elements per output field (all slots)
+  const T* kyp;           // ky of band row j (Nyquist row -> 0)
+  const cx<T>* twNY;
+  const cx<T>* twNYB;
+  int nz;
+};
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
+  using Sh = ColShape<T, NX, NY>;
+  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
+  const int f = blockIdx.z;
+  const int z = blockIdx.y + 1;
+  if (f >= 2 && z >= a.nz) return;       // w, e live on faces 1..nz-1
+  const int k0 = blockIdx.x * TK;
+  const int tid = threadIdx.x;
+  __shared__ cx<T> spec[TK * NY];
+  __shared__ cx<T> buf[2 * TK * NYB];
+
+  if (f < 3) {
+    const cx<T>* src = a.S[f] + (size_t)z * NY * MX + k0;
+    for (int i = tid; i < TK * NY; i += NT) {
+      const int c = i % TK, j = i / TK;
+      spec[c * NY + j] = src[(size_t)j * MX + c];
+    }
+    __syncthreads();
+  } else {
+    const cx<T>* src = a.Ex + (size_t)z * NY * MX + k0;
+    for (int i = tid; i < TK * NY; i += NT) {
+      const int c = i % TK, y = i / TK;
+      spec[c * NY + y] = src[(size_t)y * MX + c];
+    }
+    __syncthreads();
+    fft_run<T, NY, false, TK, NT>(spec, a.twNY, tid);
+    const T s = T(1) / T(NY);
+    for (int i = tid; i < TK * NY; i += NT) {
+      const int j = i % NY;
+      cx<T> v = spec[i];
+      spec[i] = (j == NY / 2) ? cx<T>{0, 0} : cscale(v, s);
+    }
+    __syncthreads();
+  }
+  // zero-pad the band into the 3/2 grid; second half of the batch holds i ky S
+  for (int i = tid; i < 2 * TK * NYB; i += NT) {
+    const int t = i / NYB, n = i - t * NYB;
+    const int c = t % TK;
+    int j = -1;
+    if (n < NY / 2) j = n;
+    else if (n > NYB - NY / 2) j = n - NYB + NY;
+    cx<T> v = {0, 0};
+    if (j >= 0) {
+      v = spec[c * NY + j];
+      if (t >= TK) {
+        const T k = a.kyp[j];
+        v = cx<T>{-k * v.y, k * v.x};
+      }
+    }
+    buf[i] = v;
+  }
+  __syncthreads();
+  fft_run<T, NYB, true, 2 * TK, NT>(buf, a.twNYB, tid);
+  cx<T>* o0 = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
+  cx<T>* o1 = o0 + a.bstride;
+  for (int i = tid; i < 2 * TK * NYB; i += NT) {
+    const int c = i % TK;
+    const int r = i / TK;
+    const int n = r % NYB;
+    const int d = r / NYB;
+    (d ? o1 : o0)[(size_t)n * MX + c] = buf[(d * TK + c) * NYB + n];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_rows: the pointwise tendency on the 3/2 grid, one row y per workgroup,
+// sliding over a chunk of levels.  grid (NYB, nchunks)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RowArgs {
+  const cx<T>* B;          // synthesis inputs (8 fields, see SynthArgs)
+  size_t bstride;
+  cx<T>* A;                // analysis outputs: Gu, Fyu, Gv, Fyv, Gw, Fyw, Ge, Fye
+  size_t astride;
+  const T* kxp;            // kx wavenumbers, MX entries
+  const cx<T>* twNXB;
+  const WallScalars* ws;
+  double* part;            // divz and SGS partial sums [7][slots][NYB]
+  size_t pstride;          // slots*NYB
+  int* flags;
+  Consts c;
+};
+
+template <typename T, int NX>
+struct RowCtx {
+  using Sh = RowShape<T, NX>;
+  static constexpr int MX = Sh::MX, NXB = Sh::NXB, NT = Sh::NT, NB = Sh::NB;
+};
+
+// Synthesise up to NB real row pairs.  For pair q, inputs sa[q]/sb[q] (x-spectral
+// rows, nullptr = zero row) with optional i kx factor; results in ra/rb[q][p].
+template <typename T, int NX, int NP>
+__device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, const bool* da,
+                                            const cx<T>* const* sb, const bool* db,
+                                            T (*ra)[3], T (*rb)[3], const T* __restrict__ kxp,
+                                            const cx<T>* __restrict__ tw, int tid) {
+  using C = RowCtx<T, NX>;
+  constexpr int MX = C::MX, NXB = C::NXB, NT = C::NT, NB = C::NB;
+#pragma unroll
+  for (int q0 = 0; q0 < NP; q0 += NB) {
+    constexpr int dummy = 0;
+    (void)dummy;
+#pragma unroll
+    for (int qq = 0; qq < NB; ++qq) {
+      const int q = q0 + qq;
+      cx<T>* z = buf + qq * NXB;
+      if (q < NP) {
+        const int k = tid;  // NT == MX
+        cx<T> av = {0, 0}, bv = {0, 0};
+        if (sa[q]) av = sa[q][k];
+        if (sb[q]) bv = sb[q][k];
+        const T kx = kxp[k];
+        if (da[q]) av = cx<T>{-kx * av.y, kx * av.x};
+        if (db[q]) bv = cx<T>{-kx * bv.y, kx * bv.x};
+        if (k == 0) {
+          z[0] = cx<T>{av.x, bv.x};        // c2r ignores the imaginary part of the DC term
+        } else {
+          z[k] = cx<T>{av.x - bv.y, av.y + bv.x};
+          z[NXB - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+        }
+        z[MX + k] = cx<T>{0, 0};
+        if (k == 0) z[2 * MX] = cx<T>{0, 0};
+      }
+    }
+    __syncthreads();
+    fft_run<T, NXB, true, NB, NT>(buf, tw, tid);
+#pragma unroll
+    for (int qq = 0; qq < NB; ++qq) {
+      const int q = q0 + qq;
+      if (q < NP) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const cx<T> v = buf[qq * NXB + tid + p * NT];
+          ra[q][p] = v.x;
+          rb[q][p] = v.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Forward-transform NP real row pairs; returns the first MX coefficients of
+// each member (scaled by 1/NXB) for k = tid.
+template <typename T, int NX, int NP>
+__device__ __forceinline__ void anal_pairs(cx<T>* buf, const T (*xa)[3], const T (*xb)[3],
+                                           cx<T>* oa, cx<T>* ob, const cx<T>* __restrict__ tw,
+                                           int tid) {
+  using C = RowCtx<T, NX>;
+  constexpr int NXB = C::NXB, NT = C::NT, NB = C::NB;
+  const T s = T(0.5) / T(NXB);
+#pragma unroll
+  for (int q0 = 0; q0 < NP; q0 += NB) {
+#pragma unroll
+    for (int qq = 0; qq < NB; ++qq) {
+      const int q = q0 + qq;
+      if (q < NP) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) buf[qq * NXB + tid + p * NT] = cx<T>{xa[q][p], xb[q][p]};
+      }
+    }
+    __syncthreads();
+    fft_run<T, NXB, false, NB, NT>(buf, tw, tid);
+#pragma unroll
+    for (int qq = 0; qq < NB; ++qq) {
+      const int q = q0 + qq;
+      if (q < NP) {
+        const int k = tid;
+        const cx<T> zk = buf[qq * NXB + k];
+        const cx<T> zn = buf[qq * NXB + (k == 0 ? 0 : NXB - k)];
+        oa[q] = cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s};
+        ob[q] = cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s};
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int NX>
+__global__ void __launch_bounds__(NX / 2) k_rows(RowArgs<T> a) {
+  using C = RowCtx<T, NX>;
+  constexpr int MX = C::MX, NXB = C::NXB, NT = C::NT, NB = C::NB;
+  constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
+  __shared__ cx<T> buf[NB * NXB];
+  __shared__ double red[7 * NT];
+  const int tid = threadIdx.x;
+  const int y = blockIdx.x;
+  const Consts& c = a.c;
+  const int nz = c.nz;
+  const int z0 = 1 + blockIdx.y * c.zchunk;
+  int z1 = z0 + c.zchunk;
+  if (z1 > nz + 1) z1 = nz + 1;
+  const size_t plane = (size_t)NYB * MX;
+  const size_t rowoff = (size_t)y * MX;
+  auto row = [&](int fld, int L) -> const cx<T>* {
+    return a.B + (size_t)fld * a.bstride + (size_t)L * plane + rowoff;
+  };
+  auto clampF = [&](int F) { return F <= 0 ? 1 : (F >= nz ? nz - 1 : F); };
+  enum { FU = 0, FUY, FV, FVY, FW, FWY, FE, FEY };
+
+  const T emin = (T)c.emin;
+  const T alk = (T)c.alk, xkmax = (T)c.xkmax;
+  const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
+  const T ugal = (T)c.ugal, vgal = (T)c.vgal;
+  const T c_lo = (T)a.ws->c_lo, c_hi = (T)a.ws->c_hi;
+  const T kx = a.kxp[tid];
+  int clampflag = 0;
+
+  auto xkk_of = [&](T e) {
+    T v = T(0.1) * alk * sqrt(e);
+    return v < xkmax ? v : xkmax;   // complen.f90:89
+  };
+
+  // window registers (levels iz-1, iz, iz+1 of u, v; faces of w, e)
+  T um[3], u0[3], up[3], vm[3], v0[3], vp[3], wm[3], w0[3], wp[3], em[3], e0[3], ep[3];
+  T wxm[3], wym[3];
+  {
+    // initial window: u,v at z0-1, z0 ; w,e at z0-1, z0 ; wx,wy at face z0-1
+    const cx<T>* sa[5];
+    const cx<T>* sb[5];
+    bool da[5] = {false, false, false, false, true};
+    bool db[5] = {false, false, false, false, false};
+    T ra[5][3], rb[5][3];
+    const int L0 = z0 - 1, L1 = z0;
+    sa[0] = (L0 >= 1 && L0 <= nz) ? row(FU, L0) : nullptr;
+    sb[0] = (L0 >= 1 && L0 <= nz) ? row(FV, L0) : nullptr;
+    sa[1] = row(FU, L1);
+    sb[1] = row(FV, L1);
+    sa[2] = (L0 >= 1 && L0 <= nz - 1) ? row(FW, L0) : nullptr;
+    sb[2] = row(FE, clampF(L0));
+    sa[3] = (L1 >= 1 && L1 <= nz - 1) ? row(FW, L1) : nullptr;
+    sb[3] = row(FE, clampF(L1));
+    sa[4] = (L0 >= 1 && L0 <= nz - 1) ? row(FW, L0) : nullptr;
+    sb[4] = (L0 >= 1 && L0 <= nz - 1) ? row(FWY, L0) : nullptr;
+    synth_pairs<T, NX, 5>(buf, sa, da, sb, db, ra, rb, a.kxp, a.twNXB, tid);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      um[p] = ra[0][p]; vm[p] = rb[0][p];
+      u0[p] = ra[1][p]; v0[p] = rb[1][p];
+      wm[p] = ra[2][p]; w0[p] = ra[3][p];
+      if (rb[2][p] < emin) clampflag = 1;
+      if (rb[3][p] < emin) clampflag = 1;
+      em[p] = rb[2][p] > emin ? rb[2][p] : emin;
+      e0[p] = rb[3][p] > emin ? rb[3][p] : emin;
+      wxm[p] = ra[4][p]; wym[p] = rb[4][p];
+    }
+  }
+
+  for (int iz = z0; iz < z1; ++iz) {
+    T ux[3], uy[3], vx[3], vy[3], wx0[3], wy0[3], ex[3], ey[3];
+    {
+      const cx<T>* sa[6];
+      const cx<T>* sb[6];
+      bool da[6] = {false, false, true, true, true, true};
+      bool db[6] = {false, false, false, false, false, false};
+      T ra[6][3], rb[6][3];
+      const int Lp = iz + 1;
+      sa[0] = (Lp <= nz) ? row(FU, Lp) : nullptr;
+      sb[0] = (Lp <= nz) ? row(FV, Lp) : nullptr;
+      sa[1] = (Lp <= nz - 1) ? row(FW, Lp) : nullptr;
+      sb[1] = row(FE, clampF(Lp));
+      sa[2] = row(FU, iz);
+      sb[2] = row(FUY, iz);
+      sa[3] = row(FV, iz);
+      sb[3] = row(FVY, iz);
+      sa[4] = (iz <= nz - 1) ? row(FW, iz) : nullptr;
+      sb[4] = (iz <= nz - 1) ? row(FWY, iz) : nullptr;
+      // ex, ey are not refreshed at iz = nnz-1 (derivs2.f90:279): reuse face nnz-2
+      const int ie = (iz == nz - 1) ? nz - 2 : iz;
+      sa[5] = (iz <= nz - 1) ? row(FE, ie) : nullptr;
+      sb[5] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
+      synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, a.twNXB, tid);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        up[p] = ra[0][p]; vp[p] = rb[0][p];
+        wp[p] = ra[1][p];
+        if (rb[1][p] < emin) clampflag = 1;
+        ep[p] = rb[1][p] > emin ? rb[1][p] : emin;
+        ux[p] = ra[2][p]; uy[p] = rb[2][p];
+        vx[p] = ra[3][p]; vy[p] = rb[3][p];
+        wx0[p] = ra[4][p]; wy0[p] = rb[4][p];
+        ex[p] = ra[5][p]; ey[p] = rb[5][p];
+      }
+    }
+    // ---- pointwise tendency (compadv1, complen, compsgs1, compsgs3, sufsim)
+    T fa[6][3], fb[6][3];
+    double s_div = 0, s_uu = 0, s_vv = 0, s_uv = 0, s_ww = 0, s_uw = 0, s_vw = 0;
+    const bool top = (iz == nz);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const T xm = xkk_of(em[p]), x0 = xkk_of(e0[p]), xp = xkk_of(ep[p]);
+      const T xs = xm + x0;
+      const T tuu = -xs * ux[p];                                  // compsgs1.f90:144-146
+      const T tvv = -xs * vy[p];
+      const T tuv = -xs * (uy[p] + vx[p]) * T(0.5);
+      const T wz0 = (w0[p] - wm[p]) * dzm;                        // derivs: wz at level iz
+      const T wzp = (wp[p] - w0[p]) * dzm;
+      const T tww0 = -xs * wz0;                                   // compsgs1.f90:129,151
+      const T twwp = -(x0 + xp) * wzp;
+      T tuw0, tvw0, tuwm, tvwm;
+      if (top) {                                                  // upperbc.f90:304-305
+        tuw0 = c_hi * (u0[p] + ugal);
+        tvw0 = c_hi * (v0[p] + vgal);
+      } else {                                                    // compsgs1.f90:152-153
+        tuw0 = -x0 * ((up[p] - u0[p]) * dzm + wx0[p]);
+        tvw0 = -x0 * ((vp[p] - v0[p]) * dzm + wy0[p]);
+      }
+      if (iz == 1) {                                              // lowerbc.f90:129-130
+        tuwm = -c_lo * (u0[p] + ugal);
+        tvwm = -c_lo * (v0[p] + vgal);
+      } else {
+        tuwm = -xm * ((u0[p] - um[p]) * dzm + wxm[p]);
+        tvwm = -xm * ((v0[p] - vm[p]) * dzm + wym[p]);
+      }
+      // momentum fluxes: R = d/dx Fx + d/dy Fy + H (compadv1.f90:19-51, compsgs3.f90:205-224)
+      const T Fxu = -(u0[p] * u0[p] + tuu);
+      const T Fxy = -(u0[p] * v0[p] + tuv);
+      const T Fyv = -(v0[p] * v0[p] + tvv);
+      const T Hu = T(0.016) - ((up[p] + u0[p]) * w0[p] - (um[p] + u0[p]) * wm[p]) * todzm
+                   - (tuw0 - tuwm) * dzm;
+      const T Hv = -((vp[p] + v0[p]) * w0[p] - (vm[p] + v0[p]) * wm[p]) * todzm
+                   - (tvw0 - tvwm) * dzm;
+      T Fxw = 0, Fyw = 0, Hw = 0, Fxe = 0, Fye = 0, He = 0;
+      if (!top) {
+        Fxw = -(T(0.5) * (u0[p] + up[p]) * w0[p] + tuw0);
+        Fyw = -(T(0.5) * (v0[p] + vp[p]) * w0[p] + tvw0);
+        const T wa = T(0.5) * (wp[p] + w0[p]), wb = T(0.5) * (w0[p] + wm[p]);
+        Hw = -(wa * wa - wb * wb) * dzm - (twwp - tww0) * dzm;
+        // SGS TKE: advection (compadv1.f90:59-64), diffusion and dissipation
+        // (compsgs3.f90:185-200)
+        Fxe = -T(0.5) * e0[p] * (u0[p] + up[p]) + T(2) * x0 * ex[p];
+        Fye = -T(0.5) * e0[p] * (v0[p] + vp[p]) + T(2) * x0 * ey[p];
+        He = -(wp[p] * ep[p] - wm[p] * em[p]) * todzm
+             + ((xp + x0) * (ep[p] - e0[p]) - (x0 + xm) * (e0[p] - em[p])) * dzm2
+             - ceps * e0[p] * sqrt(e0[p]);
+      }
+      fa[0][p] = Fxu; fb[0][p] = Hu;
+      fa[1][p] = Fxy; fb[1][p] = Hv;
+      fa[2][p] = Fyv; fb[2][p] = Fxw;
+      fa[3][p] = Hw;  fb[3][p] = Fyw;
+      fa[4][p] = Fxe; fb[4][p] = He;
+      fa[5][p] = Fye; fb[5][p] = T(0);
+      const double dv = (double)(ux[p] + vy[p] + wz0);            // comp1.f90:145
+      s_div += dv * dv;
+      if (c.mnout) {
+        s_uu += (double)tuu; s_vv += (double)tvv; s_uv += (double)tuv;
+        s_ww += (double)tww0; s_uw += (double)tuw0; s_vw += (double)tvw0;
+      }
+    }
+    cx<T> oa[6], ob[6];
+    anal_pairs<T, NX, 6>(buf, fa, fb, oa, ob, a.twNXB, tid);
+    {
+      const cx<T> ikx = {0, kx};
+      const size_t o = (size_t)iz * plane + rowoff + tid;
+      cx<T>* A = a.A;
+      // G = i kx Fx + H ; Fy kept for the y pass
+      A[0 * a.astride + o] = cadd(cmul(ikx, oa[0]), ob[0]);        // Gu
+      A[1 * a.astride + o] = oa[1];                                // Fyu = Fxy
+      A[2 * a.astride + o] = cadd(cmul(ikx, oa[1]), ob[1]);        // Gv
+      A[3 * a.astride + o] = oa[2];                                // Fyv
+      if (!top) {
+        A[4 * a.astride + o] = cadd(cmul(ikx, ob[2]), oa[3]);      // Gw
+        A[5 * a.astride + o] = ob[3];                              // Fyw
+        A[6 * a.astride + o] = cadd(cmul(ikx, oa[4]), ob[4]);      // Ge
+        A[7 * a.astride + o] = oa[5];                              // Fye
+      }
+    }
+    // block sums of divz and the SGS stress means
+    red[0 * NT + tid] = s_div;
+    red[1 * NT + tid] = s_uu; red[2 * NT + tid] = s_vv; red[3 * NT + tid] = s_uv;
+    red[4 * NT + tid] = s_ww; red[5 * NT + tid] = s_uw; red[6 * NT + tid] = s_vw;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+      if (tid < h) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) red[q * NT + tid] += red[q * NT + tid + h];
+      }
+      __syncthreads();
+    }
+    if (tid < 7) a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = red[tid * NT];
+    __syncthreads();
+    // slide the window
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      um[p] = u0[p]; u0[p] = up[p];
+      vm[p] = v0[p]; v0[p] = vp[p];
+      wm[p] = w0[p]; w0[p] = wp[p];
+      em[p] = e0[p]; e0[p] = ep[p];
+      wxm[p] = wx0[p]; wym[p] = wy0[p];
+    }
+  }
+  if (clampflag) atomicOr(a.flags, FLAG_ECLAMP);
+}
+
+// ---------------------------------------------------------------------------
+// k_col_anal: y-forward of the flux rows, band truncation, R = G + i ky Fy.
+// grid (MX/TK, nz, 4 components u,v,w,e)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AnalArgs {
+  const cx<T>* A;         // 8 analysis rows fields (Gu, Fyu, ...)
+  size_t astride;
+  cx<T>* R[3];            // outputs: band spectra of ru, rv, rw (comp1 result)
+  cx<T>* Tre;             // output: re as x-spectral small rows
+  const T* kyp;
+  const cx<T>* twNY;
+  const cx<T>* twNYB;
+  int nz;
+};
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
+  using Sh = ColShape<T, NX, NY>;
+  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
+  const int cmp = blockIdx.z;
+  const int z = blockIdx.y + 1;
+  const int k0 = blockIdx.x * TK;
+  const int tid = threadIdx.x;
+  __shared__ cx<T> spec[TK * NY];
+  __shared__ cx<T> buf[2 * TK * NYB];
+  const size_t splane = (size_t)NY * MX;
+  if (cmp >= 2 && z == a.nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
+    cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
+    for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
+    return;
+  }
+  const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + k0;
+  const cx<T>* fy = g + a.astride;
+  for (int i = tid; i < 2 * TK * NYB; i += NT) {
+    const int c = i % TK;
+    const int r = i / TK;
+    const int n = r % NYB;
+    const int d = r / NYB;
+    buf[(d * TK + c) * NYB + n] = (d ? fy : g)[(size_t)n * MX + c];
+  }
+  __syncthreads();
+  fft_run<T, NYB, false, 2 * TK, NT>(buf, a.twNYB, tid);
+  const T s = T(1) / T(NYB);
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, j = i / TK;
+    cx<T> v = {0, 0};
+    if (j != NY / 2) {
+      const int n = (j < NY / 2) ? j : j - NY + NYB;
+      const cx<T> gv = buf[c * NYB + n];
+      const cx<T> fv = buf[(TK + c) * NYB + n];
+      const T k = a.kyp[j];
+      v = cx<T>{(gv.x - k * fv.y) * s, (gv.y + k * fv.x) * s};
+      if (cmp == 2 && j == 0 && k0 + c == 0) v = cx<T>{0, 0};     // rw mean removal (comp1.f90:172)
+    }
+    if (cmp < 3) {
+      a.R[cmp][(size_t)z * splane + (size_t)j * MX + k0 + c] = v;
+    } else {
+      spec[c * NY + j] = v;
+    }
+  }
+  if (cmp < 3) return;
+  __syncthreads();
+  fft_run<T, NY, true, TK, NT>(spec, a.twNY, tid);
+  cx<T>* o = a.Tre + (size_t)z * splane + k0;
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, y = i / TK;
+    o[(size_t)y * MX + c] = spec[c * NY + y];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_press: substep + Poisson RHS + tridiagonal z-solve (FP64) + projection and
+// RK update.  One thread per band mode (kx, j).  grid ceil(NY*MX/256)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PressArgs {
+  cx<T>* U[3];            // U, V, W band spectra (in: stage start, out: new)
+  cx<T>* Ro[3];           // R registers (in: previous stage final, out: this stage final)
+  const cx<T>* Rn[3];     // comp1 tendencies of this stage
+  cx<double>* P;          // pressure band spectrum (out)
+  double* sd;             // scratch: reciprocal pivots [slots][modes]
+  cx<double>* sr;         // scratch: forward-eliminated rhs [slots][modes]
+  const double* kxd;      // kx wavenumbers (MX)
+  const double* kyd;      // ky wavenumbers per band row (NY)
+  int mx, ny, nz, istage;
+  double dz, dzm, dtab1, dtab2, dtabm;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<double> ld2(const cx<T>* p) {
+  const cx<T> v = *p;
+  return cx<double>{(double)v.x, (double)v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st2(cx<T>* p, cx<double> v) {
+  *p = cx<T>{(T)v.x, (T)v.y};
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_press(PressArgs<T> a) {
+  const int modes = a.mx * a.ny;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= modes) return;
+  const int j = m / a.mx, kx = m - j * a.mx;
+  if (j == a.ny / 2) return;                    // Nyquist row: always zero
+  const bool zero_mode = (j == 0 && kx == 0);
+  const double kxp = a.kxd[kx], kyp = a.kyd[j];
+  const double dz2 = a.dz * a.dz;
+  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:446-455)
+  const int nz = a.nz;
+  const size_t pl = (size_t)modes;
+  const bool sub = (a.istage != 1);
+  // forward sweep (comprhs.f90:17-35, press.F90:87-133, tridv.f90:200-209)
+  cx<double> wm_prev = {0, 0}, rw_prev = {0, 0}, r_prev = {0, 0};
+  double d_prev = 0.0;
+  for (int z = 1; z <= nz; ++z) {
+    const size_t o = (size_t)z * pl + m;
+    cx<double> u = ld2(a.U[0] + o), v = ld2(a.U[1] + o), w = ld2(a.U[2] + o);
+    if (sub) {                                   // comp1.f90:126-133
+      u = csub(u, cscale(ld2(a.Ro[0] + o), a.dtab2));
+      v = csub(v, cscale(ld2(a.Ro[1] + o), a.dtab2));
+      if (z != nz) w = csub(w, cscale(ld2(a.Ro[2] + o), a.dtab2));
+      st2(a.U[0] + o, u);
+      st2(a.U[1] + o, v);
+      st2(a.U[2] + o, w);
+    }
+    const cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
+    const cx<double> fx = cadd(ru, cscale(u, a.dtabm));
+    const cx<double> fy = cadd(rv, cscale(v, a.dtabm));
+    cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
+    const cx<double> dw = cadd(cscale(csub(w, wm_prev), a.dtabm), csub(rw, rw_prev));
+    rhs = cadd(rhs, cscale(dw, a.dzm));
+    wm_prev = w;
+    rw_prev = rw;
+    double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
+    double lo = 1.0;
+    cx<double> r = cscale(rhs, dz2);
+    if (zero_mode) {                             // press.F90:125-132
+      diag = 1.0;
+      lo = 0.0;
+      r = cx<double>{0, 0};
+    }
+    double d;
+    if (z == 1) {
+      d = 1.0 / diag;
+    } else {
+      const double fac = lo * d_prev;
+      d = 1.0 / (diag - fac * lo);               // a(i-1) = 1 (0 for the zero mode)
+      r = csub(r, cscale(r_prev, fac));
+    }
+    a.sd[o] = d;
+    a.sr[o] = r;
+    d_prev = d;
+    r_prev = r;
+  }
+  // backward sweep (tridv.f90:210-219) fused with comp2 (comp2.f90:238-263)
+  const double up_c = zero_mode ? 0.0 : 1.0;
+  cx<double> p_next = {0, 0};
+  for (int z = nz; z >= 1; --z) {
+    const size_t o = (size_t)z * pl + m;
+    const double d = a.sd[o];
+    const cx<double> r = a.sr[o];
+    cx<double> p = (z == nz) ? cscale(r, d) : cscale(csub(r, cscale(p_next, up_c)), d);
+    if (zero_mode) p = cx<double>{0, 0};         // press.F90:151-152
+    const cx<double> u = ld2(a.U[0] + o), v = ld2(a.U[1] + o), w = ld2(a.U[2] + o);
+    cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
+    ru = csub(ru, cmuli(cscale(p, kxp)));        // comp2.f90:241-242
+    rv = csub(rv, cmuli(cscale(p, kyp)));        // comp2.f90:246-247
+    cx<double> wn = {0, 0};
+    if (z != nz) {
+      rw = csub(rw, cscale(csub(p_next, p), a.dzm));   // comp2.f90:252
+      if (zero_mode) rw = cx<double>{0, 0};            // comp2.f90:253
+      wn = cadd(w, cscale(rw, a.dtab1));               // comp2.f90:260
+      if (zero_mode) wn = cx<double>{0, 0};            // comp2.f90:262
+    } else {
+      rw = cx<double>{0, 0};
+    }
+    st2(a.U[0] + o, cadd(u, cscale(ru, a.dtab1)));    // comp2.f90:257-258
+    st2(a.U[1] + o, cadd(v, cscale(rv, a.dtab1)));
+    st2(a.U[2] + o, wn);
+    st2(a.Ro[0] + o, ru);
+    st2(a.Ro[1] + o, rv);
+    st2(a.Ro[2] + o, rw);
+    a.P[o] = p;
+    p_next = p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_erows: SGS TKE update on the small grid and its x-forward transform.
+// grid (NY/2, nz); each workgroup handles rows y and y + NY/2 of level z.
+//   mode 0: re <- x-inverse(Tre); e <- max(e - dtab2 re_old + dtab1 re, emin)
+//           (comp1.f90:131, comp2.f90:261); Ex <- x-forward(e)
+//   mode 1: Ex <- x-forward(e) only (after set_state)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ERowArgs {
+  T* e;                   // physical e [slots][NY][NX]
+  T* re;                  // physical re
+  const cx<T>* Tre;       // x-spectral re of this stage
+  cx<T>* Ex;              // x-spectral e (out)
+  const cx<T>* twNX;
+  int* flags;
+  int nz, istage, mode;
+  double dtab1, dtab2, emin;
+};
+
+template <typename T, int NX>
+__global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
+  constexpr int MX = NX / 2, NT = MX, NY = NX;
+  __shared__ cx<T> buf[NX];
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x, y1 = blockIdx.x + NY / 2;
+  const int z = blockIdx.y + 1;
+  const int nz = a.nz;
+  const size_t plane = (size_t)NY * NX, splane = (size_t)NY * MX;
+  T* e = a.e + (size_t)z * plane;
+  T* re = a.re + (size_t)z * plane;
+  if (a.mode == 0 && z == nz) {                // re(nnz) = 0 (comp1.f90:176)
+    for (int p = 0; p < 2; ++p) {
+      re[(size_t)y0 * NX + tid + p * NT] = 0;
+      re[(size_t)y1 * NX + tid + p * NT] = 0;
+    }
+    return;
+  }
+  T ea[2], eb[2];
+  if (a.mode == 0) {
+    // synthesise re on rows y0 (real part) and y1 (imaginary part)
+    const cx<T>* s0 = a.Tre + (size_t)z * splane + (size_t)y0 * MX;
+    const cx<T>* s1 = a.Tre + (size_t)z * splane + (size_t)y1 * MX;
+    const int k = tid;
+    const cx<T> av = s0[k], bv = s1[k];
+    if (k == 0) {
+      buf[0] = cx<T>{av.x, bv.x};
+      buf[MX] = cx<T>{0, 0};
+    } else {
+      buf[k] = cx<T>{av.x - bv.y, av.y + bv.x};
+      buf[NX - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+    }
+    __syncthreads();
+    fft_run<T, NX, true, 1, NT>(buf, a.twNX, tid);
+    int bad = 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int x = tid + p * NT;
+      const cx<T> r = buf[x];
+      T* pe0 = e + (size_t)y0 * NX + x;
+      T* pe1 = e + (size_t)y1 * NX + x;
+      T* pr0 = re + (size_t)y0 * NX + x;
+      T* pr1 = re + (size_t)y1 * NX + x;
+      T e0 = *pe0, e1 = *pe1;
+      if (z == nz - 1) {                       // upper: e(nnz) = e(nnz-1) (upperbc.f90:197-201)
+        e[(size_t)plane + (size_t)y0 * NX + x] = e0;
+        e[(size_t)plane + (size_t)y1 * NX + x] = e1;
+      }
+      if (a.istage != 1) {
+        e0 = e0 - (T)a.dtab2 * *pr0;
+        e1 = e1 - (T)a.dtab2 * *pr1;
+      }
+      e0 = e0 + (T)a.dtab1 * r.x;
+      e1 = e1 + (T)a.dtab1 * r.y;
+      e0 = e0 > (T)a.emin ? e0 : (T)a.emin;
+      e1 = e1 > (T)a.emin ? e1 : (T)a.emin;
+      if (!(e0 == e0) || !(e1 == e1) || !isfinite(e0) || !isfinite(e1)) bad = 1;
+      *pe0 = e0; *pe1 = e1;
+      *pr0 = r.x; *pr1 = r.y;
+      ea[p] = e0; eb[p] = e1;
+    }
+    if (bad) atomicOr(a.flags, FLAG_NONFINITE);
+    __syncthreads();
+  } else {
+    if (z >= nz) return;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      ea[p] = e[(size_t)y0 * NX + tid + p * NT];
+      eb[p] = e[(size_t)y1 * NX + tid + p * NT];
+    }
+  }
+  // x-forward of the two rows
+#pragma unroll
+  for (int p = 0; p < 2; ++p) buf[tid + p * NT] = cx<T>{ea[p], eb[p]};
+  __syncthreads();
+  fft_run<T, NX, false, 1, NT>(buf, a.twNX, tid);
+  const T s = T(0.5) / T(NX);
+  const int k = tid;
+  const cx<T> zk = buf[k];
+  const cx<T> zn = buf[k == 0 ? 0 : NX - k];
+  cx<T>* o0 = a.Ex + (size_t)z * splane + (size_t)y0 * MX;
+  cx<T>* o1 = a.Ex + (size_t)z * splane + (size_t)y1 * MX;
+  o0[k] = cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s};
+  o1[k] = cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s};
+}
+
+// ---------------------------------------------------------------------------
+// Small-grid conversions used at set/get time (not on the step path)
+// ---------------------------------------------------------------------------
+// physical double rows -> x-spectral rows (x 1/NX, Nyquist dropped); grid (NY/2, nl)
+template <typename T, int NX>
+__global__ void __launch_bounds__(NX / 2) k_rows_fwd(const double* __restrict__ in, cx<T>* out,
+                                                      const cx<T>* tw, int l0) {
+  constexpr int MX = NX / 2, NT = MX, NY = NX;
+  __shared__ cx<T> buf[NX];
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x, y1 = y0 + NY / 2;
+  const int L = l0 + blockIdx.y;
+  const double* p = in + (size_t)blockIdx.y * NY * NX;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int x = tid + q * NT;
+    buf[x] = cx<T>{(T)p[(size_t)y0 * NX + x], (T)p[(size_t)y1 * NX + x]};
+  }
+  __syncthreads();
+  fft_run<T, NX, false, 1, NT>(buf, tw, tid);
+  const T s = T(0.5) / T(NX);
+  const int k = tid;
+  const cx<T> zk = buf[k], zn = buf[k == 0 ? 0 : NX - k];
+  cx<T>* o = out + (size_t)L * NY * MX;
+  o[(size_t)y0 * MX + k] = cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s};
+  o[(size_t)y1 * MX + k] = cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s};
+}
+
+// x-spectral rows -> physical double rows (unnormalised c2r); grid (NY/2, nl)
+template <typename T, int NX>
+__global__ void __launch_bounds__(NX / 2) k_rows_inv(const cx<T>* __restrict__ in, double* out,
+                                                      const cx<T>* tw, int l0) {
+  constexpr int MX = NX / 2, NT = MX, NY = NX;
+  __shared__ cx<T> buf[NX];
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x, y1 = y0 + NY / 2;
+  const int L = l0 + blockIdx.y;
+  const cx<T>* s0 = in + (size_t)L * NY * MX + (size_t)y0 * MX;
+  const cx<T>* s1 = in + (size_t)L * NY * MX + (size_t)y1 * MX;
+  const int k = tid;
+  const cx<T> av = s0[k], bv = s1[k];
+  if (k == 0) {
+    buf[0] = cx<T>{av.x, bv.x};
+    buf[MX] = cx<T>{0, 0};
+  } else {
+    buf[k] = cx<T>{av.x - bv.y, av.y + bv.x};
+    buf[NX - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+  }
+  __syncthreads();
+  fft_run<T, NX, true, 1, NT>(buf, tw, tid);
+  double* p = out + (size_t)blockIdx.y * NY * NX;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int x = tid + q * NT;
+    p[(size_t)y0 * NX + x] = (double)buf[x].x;
+    p[(size_t)y1 * NX + x] = (double)buf[x].y;
+  }
+}
+
+// x-spectral small rows <-> band spectrum along y; grid (MX/TK, nl)
+template <typename T, int NX, int NY, bool INV>
+__global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in, cx<T>* out,
+                                                     const cx<T>* tw, int l0) {
+  constexpr int MX = NX / 2, NT = 256;
+  constexpr int TK = ColShape<T, NX, NY>::TK;
+  __shared__ cx<T> buf[TK * NY];
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TK;
+  const int L = l0 + blockIdx.y;
+  const cx<T>* src = in + (size_t)L * NY * MX + k0;
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, j = i / TK;
+    cx<T> v = src[(size_t)j * MX + c];
+    if (INV && j == NY / 2) v = cx<T>{0, 0};
+    buf[c * NY + j] = v;
+  }
+  __syncthreads();
+  fft_run<T, NY, INV, TK, NT>(buf, tw, tid);
+  cx<T>* dst = out + (size_t)L * NY * MX + k0;
+  const T s = INV ? T(1) : T(1) / T(NY);
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, j = i / TK;
+    cx<T> v = cscale(buf[c * NY + j], s);
+    if (!INV && j == NY / 2) v = cx<T>{0, 0};
+    dst[(size_t)j * MX + c] = v;
+  }
+}
+
+// sums of the per-row partials: divz (comp1.f90:145) and the SGS means
+// (compsgs3.f90:236-243).  grid (slots), block 256
+__global__ void k_reduce_part(const double* __restrict__ part, size_t pstride, int nyb, int nq,
+                              double scale, double* out, int slots) {
+  __shared__ double red[256];
+  const int L = blockIdx.x;
+  for (int q = 0; q < nq; ++q) {
+    double s = 0;
+    for (int y = threadIdx.x; y < nyb; y += 256) s += part[(size_t)q * pstride + (size_t)L * nyb + y];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)q * slots + L] = red[0] * scale;
+    __syncthreads();
+  }
+}
+
+template <typename A, typename B>
+__global__ void k_cast(const A* __restrict__ in, B* out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (B)in[i];
+}
+
+}  // namespace cf
