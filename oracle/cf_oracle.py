@@ -349,7 +349,11 @@ class UtauAbort(RuntimeError):
 # ----------------------------------------------------------------------------
 # comp1: tendencies on the 3/2 grid, all levels at once (comp1.f90:1-188)
 # ----------------------------------------------------------------------------
-def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
+def sgs_fields(st: State, pr: Params):
+    """The 3/2-grid intermediates of comp1 for every level at once: expanded
+    buffers, gradients (derivsbot/derivs), eddy diffusivity (complen),
+    stresses (compsgs1) and wall stresses (sufsim/sufsim_upper).  Arrays are
+    level-offset like the state (index L+1).  Also sets st.utau/utau_s/dudt/dvdt."""
     n = pr.nnz
     nxb, nyb = pr.nnxb, pr.nnyb
     xl, yl = pr.xl, pr.yl
@@ -424,6 +428,24 @@ def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
         raise UtauAbort(f"utau={ut_hi}")
     tauuw[n + OF] = tnu; tauvw[n + OF] = tnv
     st.utau = ut_hi                                                 # sufsim_upper runs last
+
+    return dict(ub=ub, vb=vb, wb=wb, eb=eb, ux=ux, uy=uy, vx=vx, vy=vy, wz=wz, uz=uz, vz=vz,
+                wx=wx, wy=wy, ex=ex, ey=ey, xkk=xkk, tauuu=tauuu, tauvv=tauvv, tauuv=tauuv,
+                tauww=tauww, tauuw=tauuw, tauvw=tauvw)
+
+
+def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
+    n = pr.nnz
+    xl, yl = pr.xl, pr.yl
+    F = sgs_fields(st, pr)
+    ub, vb, wb, eb = F["ub"], F["vb"], F["wb"], F["eb"]
+    ux, uy, vx, vy, wz = F["ux"], F["uy"], F["vx"], F["vy"], F["wz"]
+    ex, ey, xkk = F["ex"], F["ey"], F["xkk"]
+    tauuu, tauvv, tauuv = F["tauuu"], F["tauvv"], F["tauuv"]
+    tauww, tauuw, tauvw = F["tauww"], F["tauuw"], F["tauvw"]
+    sl = slice(1 + OF, n + 1 + OF)
+    km = slice(0 + OF, n + OF)
+    k0 = slice(1 + OF, n + 1 + OF)
 
     # --- advection (compadv1.f90:3-66); levels k=1..nnz
     up = ub[2 + OF: n + 2 + OF]; u0 = ub[sl]; um = ub[0 + OF: n + OF]

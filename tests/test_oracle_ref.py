@@ -1,0 +1,111 @@
+"""Pins the oracle's SGS/wall-model layer and tridiagonal solver against the
+reference's own Fortran (complen.f90, compsgs1.f90, lowerbc.f90, upperbc.f90,
+tridv.f90, compiled in FP64 from /root/reference by oracle/build_ref.sh into
+oracle/_ref).  The reference's FFT-dependent files need FFTW 2.x, which is not
+in this image, so only this FFT-free subset is built.  Skipped when neither the
+binary nor the reference sources are present."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cf_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+BUILD = os.path.join(ROOT, "oracle", "build_ref.sh")
+
+
+@pytest.fixture(scope="module")
+def harness():
+    if not os.path.exists(HARNESS):
+        if not os.path.isdir("/root/reference"):
+            pytest.skip("reference sources absent and oracle/_ref not built")
+        r = subprocess.run(["sh", BUILD], capture_output=True, text=True)
+        if r.returncode != 0:
+            pytest.skip("oracle/_ref build failed: " + r.stderr[-500:])
+    return HARNESS
+
+
+def run(harness, payload, tmp_path):
+    fin = tmp_path / "in.bin"
+    fout = tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        for a in payload:
+            f.write(np.ascontiguousarray(a).tobytes())
+    subprocess.run([harness, str(fin), str(fout)], check=True, timeout=120)
+    return np.fromfile(fout, dtype=np.float64)
+
+
+@pytest.mark.parametrize("imodel", [3, 4])
+def test_sgs_stresses_match_reference(harness, tmp_path, imodel):
+    pr = O.Params(16, 16, 8, dt=0.01, imodel=imodel)
+    rng = np.random.default_rng(11)
+    u, v, w, e = O.initial_fields(pr, seed=5)
+    e = e * (1 + 0.5 * rng.uniform(-1, 1, e.shape))        # make xkk vary in x, y, z
+    st = O.set_state(pr, u, v, w, e)
+    st.ru[1 + O.OF: pr.nnz + 1 + O.OF] = rng.standard_normal((pr.nnz, pr.nny, pr.nnx)) * 1e-3
+    st.rv[1 + O.OF: pr.nnz + 1 + O.OF] = rng.standard_normal((pr.nnz, pr.nny, pr.nnx)) * 1e-3
+    O.lower(st, pr); O.upper(st, pr)
+    F = O.sgs_fields(st, pr)
+    n, OF = pr.nnz, O.OF
+    lv = slice(0 + OF, n + 2 + OF)      # levels 0..nnz+1
+    fc = slice(0 + OF, n + 1 + OF)      # faces 0..nnz
+    payload = [np.array([1, pr.nnx, pr.nny, pr.nnz, imodel, pr.isurface], dtype=np.int32),
+               np.array([pr.xl, pr.yl, pr.zl, pr.z0, pr.dt, pr.alk], dtype=np.float64)]
+    payload += [F[k][lv] for k in ("ux", "uy", "vx", "vy", "wz")]
+    payload += [F[k][fc] for k in ("uz", "vz", "wx", "wy", "eb")]
+    payload += [F["ub"][1 + OF], F["ub"][n + OF], F["vb"][1 + OF], F["vb"][n + OF]]
+    payload += [st.ru[1 + OF: n + 1 + OF], st.rv[1 + OF: n + 1 + OF]]
+    out = run(harness, payload, tmp_path)
+    P = pr.nnxb * pr.nnyb
+    shp = (pr.nnyb, pr.nnxb)
+    pos = 0
+
+    def take(m=1):
+        nonlocal pos
+        a = out[pos: pos + m * P].reshape((m,) + shp) if m > 1 else out[pos: pos + P].reshape(shp)
+        pos += m * P
+        return a
+
+    def close(a, b, what):
+        scale = max(np.abs(b).max(), 1e-30)
+        assert np.abs(a - b).max() <= 1e-12 * scale, (what, np.abs(a - b).max(), scale)
+
+    close(take(), F["tauuw"][0 + OF], "tauuw face 0 (sufsim)")
+    close(take(), F["tauvw"][0 + OF], "tauvw face 0 (sufsim)")
+    close(take(), F["xkk"][0 + OF], "xkk face 0")
+    for iz in range(1, n + 1):
+        a = take(7)
+        close(a[0], F["tauuu"][iz - 1], f"tauuu {iz}")
+        close(a[1], F["tauvv"][iz - 1], f"tauvv {iz}")
+        close(a[2], F["tauuv"][iz - 1], f"tauuv {iz}")
+        close(a[3], F["tauww"][iz + OF], f"tauww {iz}")
+        close(a[4], F["tauuw"][iz + OF], f"tauuw face {iz}")
+        close(a[5], F["tauvw"][iz + OF], f"tauvw face {iz}")
+        close(a[6], F["xkk"][iz + OF], f"xkk face {iz}")
+        if iz == 1:
+            sc = out[pos: pos + 4]
+            pos += 4
+            # utau of the lower wall, utau_s, dudt, dvdt (lowerbc.f90:67-72)
+            u1 = F["ub"][1 + OF].mean(); v1 = F["vb"][1 + OF].mean()
+            vs = max(np.hypot(u1, v1), 0.01)
+            assert abs(sc[0] - vs * O.VK / pr.zody) < 1e-14
+            assert abs(sc[1] - st.utau_s) < 1e-14
+            assert abs(sc[2] - st.dudt) < 1e-17 and abs(sc[3] - st.dvdt) < 1e-17
+    assert abs(out[pos] - st.utau) < 1e-14      # sufsim_upper sets the final utau
+    assert pos + 1 == out.size
+
+
+def test_tridv_matches_reference(harness, tmp_path):
+    rng = np.random.default_rng(2)
+    ny, nz = 9, 13
+    b = rng.uniform(0.5, 1.5, (nz, ny))
+    a = rng.uniform(0.5, 1.5, (nz, ny))
+    d = -(a + b + rng.uniform(0.1, 1, (nz, ny)))
+    r = rng.standard_normal((nz, ny))
+    bdar = np.stack([b, d, a, r])             # Fortran bdar(ny, nz, 4): C order (4, nz, ny)
+    out = run(harness, [np.array([2, ny, nz], dtype=np.int32), bdar], tmp_path).reshape(nz, ny)
+    mine = O.tridv(b, d, a, r)
+    assert np.abs(out - mine).max() < 1e-14 * np.abs(mine).max()
