@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the channel-flow RK3 timestep on one MI355X.
+
+Metric: grid-point RK3 steps per second (nnx*nny*nnz*steps / time) on the
+256x256x128 channel at Re_tau = 395 (z0 = exp(-0.4*5)/395), seeded log-law +
+0.1 u_tau noise, TKE SGS closure.  ``value`` is the FP32 path (FP32 fields,
+transforms, products and SGS; FP64 Poisson RHS, tridiagonal solve, projection
+and update); the FP64 plain path is timed beside it.  Fields are resident in
+HBM before the timed region; the timed region is K full steps on the library
+stream bracketed by stream synchronisation, measured with HIP events.
+
+One JSON line is printed.  ``roofline`` prices the step at the algorithmic
+HBM bytes per grid-point step (FP32 path: 85 FP32 + 12 FP64 array passes =
+436 B; FP64 path: 97 x 8 = 776 B) against the 8 TB/s HBM3E peak, and names the
+dominant kernel with its measured per-launch time.  ``cpu_baseline`` times
+the FP64 numpy oracle (the CPU restatement of the reference stage) on one
+stage of the same 256x256x128 workload, on one host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "paes-channelflow_amd"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_GPSTEP = {"fp32": 436.0, "fp64": 776.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--re-tau", type=float, default=395.0)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain path")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-nz", type=int, default=0,
+                    help="levels of the CPU-baseline sample (default: the full nz)")
+    return ap.parse_args()
+
+
+def run_gpu(args, precision_name):
+    import numpy as np
+    import channelflow as C
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    nx, nz = args.nx, args.nz
+    z0 = C.z0_of_retau(args.re_tau)
+    dz = 2.0 / nz
+    # fixed dt at CFL ~ 0.3 of the 3/2 grid for |u| ~ 3 u_tau/kappa-scale velocities
+    dt = 0.3 / (3.2 * (3 * nx / 2) / (2 * math.pi) + 0.5 / dz)
+    prec = C.CF_PREC_FP32 if precision_name == "fp32" else C.CF_PREC_FP64
+    rp = C.RunParams(nx, nx, nz, z0=z0, dt=dt, precision=prec)
+    g = C.ChannelFlow(rp)
+    u, v, w, e = seeded_fields(nx, nz, z0)
+    g.set_state(u, v, w, e)
+    if args.warmup > 0:
+        g.step(args.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    ms = g.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    _, per_kernel = g.time_steps(1, per_kernel=True)
+    g.sync()
+    st = g.get_state()
+    finite = all(bool(np.isfinite(st[k]).all()) for k in "uvwe")
+    out = dict(ms=ms, wall=wall, per_kernel=per_kernel, dt=dt, z0=z0, finite=finite,
+               device_bytes=g.device_bytes())
+    g.close()
+    return out
+
+
+def seeded_fields(nx, nz, z0):
+    """The seeded log-law + noise initial condition (the oracle's recipe,
+    computed here directly so bench does not import the oracle)."""
+    import numpy as np
+    ut = math.sqrt(0.016 * 1.0)
+    zl = 2.0
+    dz = zl / nz
+    zm = (np.arange(1, nz + 1) - 0.5) * dz
+    zw = np.minimum(zm, zl - zm)
+    prof = ut / 0.4 * np.log(zw / z0)
+    rng = np.random.default_rng(10972)
+    shp = (nz, nx, nx)
+    u = prof[:, None, None] + 0.1 * ut * rng.uniform(-1, 1, shp)
+    v = 0.1 * ut * rng.uniform(-1, 1, shp)
+    w = 0.1 * ut * rng.uniform(-1, 1, shp)
+    w[-1] = 0.0
+    e = np.full(shp, 0.01 * ut * ut)
+    return u, v, w, e
+
+
+def cpu_baseline(args):
+    """FP64 numpy oracle (the CPU restatement) on one RK3 stage of the same
+    grid; reported in grid-point RK3 steps/s = points / (3 * stage time)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cf_oracle as O
+    nz = args.cpu_sample_nz or args.nz
+    pr = O.Params(args.nx, args.nx, nz, z0=math.exp(-2.0) / args.re_tau, dt=0.002)
+    st = O.set_state(pr, *O.initial_fields(pr))
+    t0 = time.perf_counter()
+    O.stage(st, pr, 1)
+    t = time.perf_counter() - t0
+    pts = args.nx * args.nx * nz
+    return {"value": pts / (3.0 * t), "unit": "grid-pt RK3 steps/s", "cores": 1, "kind": "port",
+            "sample": f"one RK3 stage (x3 for a step) of the FP64 numpy oracle at "
+                      f"{args.nx}x{args.nx}x{nz}, {t:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU z-slab design is described, not built)"}))
+        sys.exit(2)
+    pts = args.nx * args.nx * args.nz
+    print(f"[bench] fp32 path {args.nx}x{args.nx}x{args.nz}", file=sys.stderr, flush=True)
+    r32 = run_gpu(args, "fp32")
+    value = pts * args.steps / (r32["ms"] * 1e-3)
+    print("[bench] fp64 path", file=sys.stderr, flush=True)
+    r64 = None if args.no_fp64 else run_gpu(args, "fp64")
+    kern = r32["per_kernel"]
+    dom = max((k for k in kern if not k.startswith("(")), key=lambda k: kern[k])
+    step_ms = r32["ms"] / args.steps
+    achieved = BYTES_PER_GPSTEP["fp32"] * value / 1e9
+    line = {
+        "metric": "grid-pt RK3 steps/s @256^2x128: FP32(+FP64 pressure) TKE-SGS path; % HBM peak",
+        "value": value,
+        "unit": "grid-pt RK3 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 pressure/projection)",
+        "data": "synthetic: seeded log-law + 0.1 u_tau uniform noise, e = 0.01 u_tau^2",
+        "config": {
+            "workload": f"channel {args.nx}x{args.nx}x{args.nz}, 2pi x pi x 2, Re_tau={args.re_tau:g} "
+                        f"(z0={r32['z0']:.4e}), TKE SGS (imodel 3), Schumann wall (isurface 1), "
+                        f"3/2 dealiasing, dt={r32['dt']:.3e}",
+            "grid": [args.nx, args.nx, args.nz],
+            "precision": "fp32 fields/FFT/SGS, fp64 pressure",
+            "learned_sgs": False,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "basis": "436 algorithmic HBM bytes per grid-point RK3 step x grid-pt steps/s, "
+                     "time from HIP events over the step loop",
+            "dominant_kernel": dom,
+            "dominant_kernel_ms_per_launch": kern[dom],
+            "kernel_ms_per_stage": kern,
+        },
+        "fp64_plain": None,
+        "finite": r32["finite"],
+    }
+    if r64 is not None:
+        v64 = pts * args.steps / (r64["ms"] * 1e-3)
+        line["fp64_plain"] = {
+            "value": v64, "ms_per_step": r64["ms"] / args.steps,
+            "hbm_frac": BYTES_PER_GPSTEP["fp64"] * v64 / 1e9 / HBM_PEAK_GBS,
+            "fp32_over_fp64": value / v64, "kernel_ms_per_stage": r64["per_kernel"],
+            "finite": r64["finite"],
+        }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (numpy oracle, one stage)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -77,6 +77,14 @@ int cf_destroy(cf_ctx* ctx);
 int cf_set_state(cf_ctx* ctx, const double* u, const double* v, const double* w,
                  const double* e);
 
+/* Seeded initial condition, written to the device as by cf_set_state.
+ * u = u_tau/0.4 ln(z_w/z0) + amp u_tau xi, v = amp u_tau xi, w = amp u_tau xi
+ * (faces 1..nnz-1; w(nnz) = 0), e = 0.01 u_tau^2, with u_tau = sqrt(0.016 zl/2),
+ * z_w = min(zm, zl-zm), zm = (iz-1/2) dz (the log profile of init.f90:484-491)
+ * and xi in [-1,1) from SplitMix64 keyed by (seed, field, i, j, iz).  Replaces
+ * initvel (init.f90:463-527), whose random stream is compiler-specific. */
+int cf_seeded_init(cf_ctx* ctx, uint64_t seed, double amp);
+
 /* Read the interior levels 1..nnz of u, v, w, e and p (any pointer may be
  * NULL).  Replaces the per-level record of sav_field (read_write.f90:117-126). */
 int cf_get_state(cf_ctx* ctx, double* u, double* v, double* w, double* e, double* p);

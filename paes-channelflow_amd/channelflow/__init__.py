@@ -35,7 +35,7 @@ CF_PREC_FP32 = 1
 CF_NKERN = 8
 
 EXPORTS = (
-    "cf_create", "cf_destroy", "cf_set_state", "cf_get_state", "cf_get_tendency",
+    "cf_create", "cf_destroy", "cf_set_state", "cf_seeded_init", "cf_get_state", "cf_get_tendency",
     "cf_get_comp1_tendency", "cf_stage", "cf_step", "cf_set_dt", "cf_sync", "cf_get_divz",
     "cf_get_scalars", "cf_compmn", "cf_time_steps", "cf_kernel_name", "cf_device_bytes",
     "cf_last_error",
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     lib.cf_create.argtypes = [ctypes.POINTER(CfParams), ctypes.POINTER(vp)]
     lib.cf_destroy.argtypes = [vp]
     lib.cf_set_state.argtypes = [vp, dp, dp, dp, dp]
+    lib.cf_seeded_init.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
     lib.cf_get_state.argtypes = [vp, dp, dp, dp, dp, dp]
     lib.cf_get_tendency.argtypes = [vp, dp, dp, dp, dp]
     lib.cf_get_comp1_tendency.argtypes = [vp, dp, dp, dp]
@@ -180,6 +181,9 @@ class ChannelFlow:
     def set_state(self, u, v, w, e):
         u, v, w, e = (self._arr(x) for x in (u, v, w, e))
         self._check(self.lib.cf_set_state(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(e)))
+
+    def seeded_init(self, seed: int = 10972, amp: float = 0.1):
+        self._check(self.lib.cf_seeded_init(self._h, seed, amp))
 
     def get_state(self):
         out = [np.empty(self.shape) for _ in range(5)]

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -536,6 +537,41 @@ int cf_set_state(cf_ctx* ctx, const double* u, const double* v, const double* w,
   HIPCHK(hipSetDevice(ctx->p.device));
   HIPCHK(hipMemset(&ctx->ws->flags, 0, sizeof(int)));
   return ctx->eng->set_state(ctx, u, v, w, e);
+}
+
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+int cf_seeded_init(cf_ctx* ctx, uint64_t seed, double amp) {
+  if (!ctx) return CF_E_ARG;
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  const size_t pl = (size_t)nx * ny;
+  std::vector<double> f[4];
+  for (auto& a : f) a.resize((size_t)nz * pl);
+  const double zl = ctx->p.zl, dz = zl / nz;
+  const double ut = std::sqrt(0.016 * zl / 2.0);
+  const uint64_t key0 = seed * 0x100000001B3ull;
+  for (int fld = 0; fld < 3; ++fld)
+    for (int k = 1; k <= nz; ++k) {
+      const double zm = (k - 0.5) * dz;
+      const double zw = zm < zl - zm ? zm : zl - zm;
+      const double prof = (fld == 0) ? ut / 0.4 * std::log(zw / ctx->p.z0) : 0.0;
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const uint64_t key = key0 ^ ((uint64_t)fld << 56) ^ ((uint64_t)k << 40) ^
+                               ((uint64_t)j << 20) ^ (uint64_t)i;
+          const double xi = (double)(splitmix64(key) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+          double val = prof + amp * ut * xi;
+          if (fld == 2 && k == nz) val = 0.0;
+          f[fld][(size_t)(k - 1) * pl + (size_t)j * nx + i] = val;
+        }
+    }
+  std::fill(f[3].begin(), f[3].end(), 0.01 * ut * ut);
+  return cf_set_state(ctx, f[0].data(), f[1].data(), f[2].data(), f[3].data());
 }
 
 static int get_fields(cf_ctx* ctx, const int* which, double* const* outs, int n) {

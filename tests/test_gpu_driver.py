@@ -1,0 +1,81 @@
+"""The Fortran driver les_gpu (drop-in for les.F90) on the GPU: reads an
+nlist.input, runs the time loop through the C-ABI and writes .field/.mean
+files that match the FP64 oracle run from the same initial state.  The REAL(4)
+restart path is exercised as the shipped reference writes it."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cf_oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "paes-channelflow_amd", "fortran", "les_gpu")
+
+NML = """&RESTARTING restart={restart}, restart_file='start.field' /
+&TIMING case='chan', timax=1, savfrq=1, meanfrq=0.5, isbgr=.true., dt={dt}, imodel=3, isurface=1 /
+&GRIDSIZE nnx=32, nny=32, nnz=16 /
+&PARAMS xl=6.283185307179586, yl=3.141592653589793, zl=2.0, z0=7.519e-4 /
+&GPU precision=0, seed=10972, amp=0.1 /
+"""
+
+
+def run_driver(tmp_path, restart, dt):
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built")
+    (tmp_path / "nlist.input").write_text(NML.format(restart=".true." if restart else ".false.", dt=dt))
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def test_driver_seeded_run_matches_oracle(tmp_path):
+    from channelflow.fieldio import read_field
+    dt = 1.0 / 32
+    out = run_driver(tmp_path, False, dt)
+    its = [int(m) for m in re.findall(r"Iteration\s+(\d+)\s*;", out)]
+    assert its == list(range(1, 33)), its[-3:]
+    hdr, f = read_field(tmp_path / "chan1.field")
+    assert hdr["kind"] == 8 and hdr["nnz"] == 16 and abs(hdr["time"] - 1.0) < 1e-12
+    pr = O.Params(32, 32, 16, dt=dt, z0=7.519e-4)
+    st = O.set_state(pr, *O.initial_fields(pr))
+    for it in range(32):
+        O.step(st, pr, mnout=(it in (0, 15, 31)))
+    for k in "uvwe":
+        ref = O.interior(st, pr, k)
+        assert np.linalg.norm(f[k] - ref) / np.linalg.norm(ref) < 1e-10, k
+    assert abs(hdr["utau"] - st.utau) < 1e-12
+    # .mean at t = 1.00: nnz, time / utau, z0 / zm, zt / englez, engsbz / means ...
+    vals = np.array((tmp_path / "chan1.00.mean").read_text().split(), dtype=float)
+    n = 16
+    mb = O.compmn(st, pr)
+    body = vals[4:]
+    zm, zt = body[:n], body[n:2 * n]
+    assert np.allclose(zm, (np.arange(1, n + 1) - 0.5) * pr.dz) and np.allclose(zt, np.arange(1, n + 1) * pr.dz)
+    englez, engsbz = body[2 * n:3 * n], body[3 * n:4 * n]
+    uxym = body[4 * n:5 * n]
+    assert np.allclose(englez, mb[0], rtol=1e-8) and np.allclose(engsbz, mb[1], rtol=1e-10)
+    assert np.allclose(uxym, mb[3], rtol=1e-10)
+    assert (tmp_path / "parameters.dat").read_text().startswith("       #Time")
+
+
+def test_driver_restart_from_real4_file(tmp_path):
+    from channelflow.fieldio import write_field, read_field
+    pr = O.Params(32, 32, 16, dt=1.0 / 32, z0=7.519e-4)
+    u, v, w, e = (a.astype(np.float32).astype(np.float64) for a in O.initial_fields(pr))
+    hdr = dict(time=0.0, nnx=32, nny=32, nnz=16, xl=pr.xl, yl=pr.yl, zl=pr.zl, dt=pr.dt, z0=pr.z0)
+    write_field(tmp_path / "start.field", hdr, dict(u=u, v=v, w=w, e=e, p=np.zeros_like(u)), kind=4)
+    run_driver(tmp_path, True, pr.dt)
+    # restart header dt/z0 are REAL(4); the oracle uses the same rounded values
+    pr = O.Params(32, 32, 16, dt=float(np.float32(pr.dt)), z0=float(np.float32(pr.z0)),
+                  xl=float(np.float32(pr.xl)), yl=float(np.float32(pr.yl)), zl=2.0)
+    st = O.set_state(pr, u, v, w, e)
+    for it in range(32):
+        O.step(st, pr)
+    _, f = read_field(tmp_path / "chan1.field")
+    for k in "uvwe":
+        ref = O.interior(st, pr, k)
+        assert np.linalg.norm(f[k] - ref) / np.linalg.norm(ref) < 1e-10, k
