@@ -6,14 +6,15 @@
 // transforms that sits contiguously in LDS: stage s of radix R reads the R
 // inputs of each butterfly into registers, applies the twiddles
 // w_N^(r*k*N/(Ns*R)), does the R-point DFT, waits at a barrier, and writes the
-// outputs back in place at (j/Ns)*Ns*R + k + r*Ns.  Radix 4 is used while the
-// power of two allows, then one radix-2 stage, then radix 3.
+// outputs back in place at (j/Ns)*Ns*R + k + r*Ns.  Radix 8 is used while the
+// power of two allows, then radix 2/4, or radix 6 for a lone factor 2 with
+// the factor 3 (384 = 8*8*6: three stages, six barriers).
 //
 // Sign convention: forward = exp(-2 pi i nk/N) (the sign FFTW 2.x uses for
 // rfftw forward plans, filter_fftw.f90:49-74), unnormalised in both
 // directions; callers apply 1/N where the reference does (ffthrz's nxym1,
 // xderiv's nxm1).  Twiddles come from a table tw[m] = exp(-2 pi i m/N)
-// computed on the host in double precision.
+// computed on the host in double precision and staged in LDS by the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -55,16 +56,28 @@ constexpr int pow3_of(int n) {
   while (n % 3 == 0) { n /= 3; ++b; }
   return b;
 }
-constexpr int n_stages(int n) { return pow2_of(n) / 2 + pow2_of(n) % 2 + pow3_of(n); }
+// Stage radices: 8 while at least three factors of two remain, then the rest
+// of the power of two (2 or 4), fused with a factor 3 into radix 6 when a
+// single 2 is left: 384 = 8*8*6, 256 = 8*8*4, 768 = 8*8*4*3, 48 = 8*6.
+constexpr int n_stages(int n) {
+  const int a = pow2_of(n), b = pow3_of(n);
+  const int r = a % 3;
+  int s = a / 3;
+  if (b == 1) s += (r == 1) ? 1 : (r == 2 ? 2 : 1);
+  else s += (r != 0) ? 1 : 0;
+  return s;
+}
 constexpr int radix_of(int n, int s) {
-  const int q = pow2_of(n) / 2;
-  if (s < q) return 4;
+  const int a = pow2_of(n), b = pow3_of(n);
+  const int q = a / 3, r = a % 3;
+  if (s < q) return 8;
   s -= q;
-  if (pow2_of(n) % 2) {
-    if (s == 0) return 2;
-    s -= 1;
+  if (b == 1) {
+    if (r == 1) return 6;
+    if (r == 2) return s == 0 ? 4 : 3;
+    return 3;
   }
-  return 3;
+  return r == 1 ? 2 : 4;
 }
 constexpr int span_before(int n, int s) {
   int p = 1;
@@ -88,7 +101,7 @@ __device__ __forceinline__ void butterfly(cx<T>* v) {
     v[0] = cadd(v[0], t1);
     v[1] = cadd(t2, t3);
     v[2] = csub(t2, t3);
-  } else {  // R == 4
+  } else if constexpr (R == 4) {
     cx<T> a = cadd(v[0], v[2]);
     cx<T> b = csub(v[0], v[2]);
     cx<T> c = cadd(v[1], v[3]);
@@ -99,12 +112,73 @@ __device__ __forceinline__ void butterfly(cx<T>* v) {
     v[1] = cadd(b, d);
     v[2] = csub(a, c);
     v[3] = csub(b, d);
+  } else if constexpr (R == 6) {
+    // two 3-point DFTs (even, odd samples) joined by w6^k = exp(-+ i pi k/3)
+    const T s3 = T(0.86602540378443864676372317075294);
+    cx<T> a[3] = {v[0], v[2], v[4]};
+    cx<T> b[3] = {v[1], v[3], v[5]};
+    butterfly<T, 3, INV>(a);
+    butterfly<T, 3, INV>(b);
+    // w6 = 0.5 -+ i s3 ; w6^2 = -0.5 -+ i s3
+    const cx<T> b1 = INV ? cx<T>{T(0.5) * b[1].x - s3 * b[1].y, T(0.5) * b[1].y + s3 * b[1].x}
+                         : cx<T>{T(0.5) * b[1].x + s3 * b[1].y, T(0.5) * b[1].y - s3 * b[1].x};
+    const cx<T> b2 = INV ? cx<T>{-T(0.5) * b[2].x - s3 * b[2].y, -T(0.5) * b[2].y + s3 * b[2].x}
+                         : cx<T>{-T(0.5) * b[2].x + s3 * b[2].y, -T(0.5) * b[2].y - s3 * b[2].x};
+    v[0] = cadd(a[0], b[0]);
+    v[3] = csub(a[0], b[0]);
+    v[1] = cadd(a[1], b1);
+    v[4] = csub(a[1], b1);
+    v[2] = cadd(a[2], b2);
+    v[5] = csub(a[2], b2);
+  } else {  // R == 8: two 4-point DFTs joined by w8^k
+    const T h = T(0.70710678118654752440084436210485);
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    butterfly<T, 4, INV>(e);
+    butterfly<T, 4, INV>(o);
+    const cx<T> o1 = INV ? cx<T>{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)}
+                         : cx<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+    const cx<T> o2 = INV ? cx<T>{-o[2].y, o[2].x} : cx<T>{o[2].y, -o[2].x};
+    const cx<T> o3 = INV ? cx<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)}
+                         : cx<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
   }
 }
 
-// One Stockham stage over NBT transforms of length N held at a[t*N + n].
-// Entry and exit are barrier-synchronised.
-template <typename T, int N, bool INV, int NBT, int NT, int S>
+// Copy the length-n twiddle table into LDS (call before the first fft_run;
+// the caller's next barrier publishes it).
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* __restrict__ src, int n,
+                                              int tid, int nt) {
+  for (int i = tid; i < n; i += nt) dst[i] = src[i];
+}
+
+// Ordering point for LDS traffic among the lanes of one wavefront: LDS
+// instructions of a wave execute in order, so a compiler-level fence suffices
+// (no s_barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync() {
+  if constexpr (WAVE) wave_sync();
+  else __syncthreads();
+}
+
+// One Stockham stage over NBT transforms of length N held at a[t*N + n],
+// executed by NT threads (a whole workgroup, or one wavefront of 64 lanes
+// when WAVE).  Entry and exit are synchronised at that scope.
+template <typename T, int N, bool INV, int NBT, int NT, int S, bool WAVE>
 __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
   constexpr int R = radix_of(N, S);
   constexpr int NS = span_before(N, S);
@@ -133,7 +207,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
       butterfly<T, R, INV>(v[q]);
     }
   }
-  __syncthreads();
+  fft_sync<WAVE>();
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int idx = tid + q * NT;
@@ -146,14 +220,28 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
       for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
     }
   }
-  __syncthreads();
+  fft_sync<WAVE>();
 }
 
-template <typename T, int N, bool INV, int NBT, int NT, int S = 0>
+template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false>
 __device__ __forceinline__ void fft_run(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
   if constexpr (S < n_stages(N)) {
-    fft_stage<T, N, INV, NBT, NT, S>(a, tw, tid);
-    fft_run<T, N, INV, NBT, NT, S + 1>(a, tw, tid);
+    fft_stage<T, N, INV, NBT, NT, S, WAVE>(a, tw, tid);
+    fft_run<T, N, INV, NBT, NT, S + 1, WAVE>(a, tw, tid);
+  }
+}
+
+// NBT transforms spread over the NW wavefronts of a workgroup, each wave
+// running its share with wave-level ordering only.  The caller synchronises
+// the workgroup before (inputs written) and after (outputs read by others).
+template <typename T, int N, bool INV, int NBT, int NW>
+__device__ __forceinline__ void fft_waves(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
+  const int w = tid >> 6, lane = tid & 63;
+  if constexpr (NBT % NW == 0) {
+    constexpr int PW = NBT / NW;
+    fft_run<T, N, INV, PW, 64, 0, true>(a + w * PW * N, tw, lane);
+  } else {
+    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true>(a + t * N, tw, lane);
   }
 }
 

@@ -49,9 +49,8 @@ template <typename T, int NX>
 struct RowShape {
   static constexpr int MX = NX / 2;
   static constexpr int NXB = 3 * NX / 2;
-  static constexpr int NT = MX;          // threads; each owns 3 points of a 3/2-grid row
-  static constexpr int PPT = 3;
-  static constexpr int NB = (6 * NXB * (int)sizeof(cx<T>) <= 65536) ? 6 : 3;  // transforms per batch
+  static constexpr int NT = NXB;         // one thread per 3/2-grid point of the row
+  static constexpr int NB = 6;           // transforms per LDS batch
 };
 
 // Device-side scalars shared by the stage kernels.
@@ -74,15 +73,77 @@ struct Consts {
   int nz, istage, mnout, zchunk;
 };
 
+// Sum of NQ per-thread values over the workgroup; results land in out[q] of
+// thread 0..NQ-1 (returned through red, an LDS scratch of NQ*NT doubles).
+template <int NT, int NQ>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double* red, int tid) {
+  if constexpr (NT % 64 == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+    constexpr int NW = NT / 64;
+    if ((tid & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) red[q * NW + tid / 64] = v[q];
+    }
+    __syncthreads();
+    if (tid < NQ) {
+      double s = 0;
+      for (int w = 0; w < NW; ++w) s += red[tid * NW + w];
+      red[NQ * NW + tid] = s;
+    }
+    __syncthreads();
+    if (tid < NQ) v[0] = red[NQ * NW + tid];
+  } else {
+    constexpr int RP = (NT <= 64) ? 64 : (NT <= 128 ? 128 : (NT <= 256 ? 256 : (NT <= 512 ? 512 : 1024)));
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      red[q * RP + tid] = v[q];
+      if (tid + NT < RP) red[q * RP + tid + NT] = 0.0;
+    }
+    __syncthreads();
+    for (int h = RP / 2; h > 0; h >>= 1) {
+      if (tid < h) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) red[q * RP + tid] += red[q * RP + tid + h];
+      }
+      __syncthreads();
+    }
+    if (tid < NQ) v[0] = red[tid * RP];
+  }
+  __syncthreads();
+}
+
+template <int NT, int NQ>
+struct RedSize {
+  static constexpr int RP = (NT <= 64) ? 64 : (NT <= 128 ? 128 : (NT <= 256 ? 256 : (NT <= 512 ? 512 : 1024)));
+  static constexpr int N = (NT % 64 == 0) ? NQ * (NT / 64 + 1) : NQ * RP;
+};
+
 // ---------------------------------------------------------------------------
-// k_wall: wall-model scalars (one thread)
+// k_wall: wall-model scalars.  One workgroup of 64 threads.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_wall(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
-                       const cx<T>* __restrict__ Ruo, const cx<T>* __restrict__ Rvo,
-                       size_t plane, Consts c, WallScalars* ws) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
+                                             const cx<T>* __restrict__ Ruo,
+                                             const cx<T>* __restrict__ Rvo, size_t plane, Consts c,
+                                             WallScalars* ws) {
   const int nz = c.nz;
+  const int tid = threadIdx.x;
+  // dudt, dvdt: volume means of the previous stage's R (lowerbc.f90:70-72);
+  // the plane sum of a band field is nx*ny times its (0,0) coefficient
+  double su = 0.0, sv = 0.0;
+  for (int z = 1 + tid; z <= nz; z += 64) {
+    su += (double)Ruo[(size_t)z * plane].x;
+    sv += (double)Rvo[(size_t)z * plane].x;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    su += __shfl_xor(su, off, 64);
+    sv += __shfl_xor(sv, off, 64);
+  }
+  if (tid != 0) return;
   // plane means = the (0,0) coefficient (lowerbc.f90:57-58)
   double u1 = (double)U[1 * plane].x, v1 = (double)V[1 * plane].x;
   double vs = sqrt(u1 * u1 + v1 * v1);
@@ -98,11 +159,6 @@ __global__ void k_wall(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
   ws->utau_hi = ut2;
   ws->c_hi = ut2 * ut2 / vs2;
   ws->utau = ut2;                                               // sufsim_upper runs last
-  double su = 0.0, sv = 0.0;                                    // lowerbc.f90:70-72
-  for (int z = 1; z <= nz; ++z) {
-    su += (double)Ruo[(size_t)z * plane].x;
-    sv += (double)Rvo[(size_t)z * plane].x;
-  }
   ws->dudt = su / nz;
   ws->dvdt = sv / nz;
   ws->utau_s = sqrt(c.zl / 2.0 * sqrt(ws->dvdt * ws->dvdt + (0.016 - ws->dudt) * (0.016 - ws->dudt)));
@@ -136,6 +192,9 @@ __global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
   const int tid = threadIdx.x;
   __shared__ cx<T> spec[TK * NY];
   __shared__ cx<T> buf[2 * TK * NYB];
+  __shared__ cx<T> twb[NYB];
+  __shared__ cx<T> tws[NY];
+  load_twiddles(twb, a.twNYB, NYB, tid, NT);
 
   if (f < 3) {
     const cx<T>* src = a.S[f] + (size_t)z * NY * MX + k0;
@@ -145,13 +204,15 @@ __global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
     }
     __syncthreads();
   } else {
+    load_twiddles(tws, a.twNY, NY, tid, NT);
     const cx<T>* src = a.Ex + (size_t)z * NY * MX + k0;
     for (int i = tid; i < TK * NY; i += NT) {
       const int c = i % TK, y = i / TK;
       spec[c * NY + y] = src[(size_t)y * MX + c];
     }
     __syncthreads();
-    fft_run<T, NY, false, TK, NT>(spec, a.twNY, tid);
+    fft_waves<T, NY, false, TK, NT / 64>(spec, tws, tid);
+    __syncthreads();
     const T s = T(1) / T(NY);
     for (int i = tid; i < TK * NY; i += NT) {
       const int j = i % NY;
@@ -178,7 +239,8 @@ __global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
     buf[i] = v;
   }
   __syncthreads();
-  fft_run<T, NYB, true, 2 * TK, NT>(buf, a.twNYB, tid);
+  fft_waves<T, NYB, true, 2 * TK, NT / 64>(buf, twb, tid);
+  __syncthreads();
   cx<T>* o0 = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
   cx<T>* o1 = o0 + a.bstride;
   for (int i = tid; i < 2 * TK * NYB; i += NT) {
@@ -192,7 +254,7 @@ __global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // k_rows: the pointwise tendency on the 3/2 grid, one row y per workgroup,
-// sliding over a chunk of levels.  grid (NYB, nchunks)
+// sliding over a chunk of levels.  grid (NYB, nchunks), one thread per point.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct RowArgs {
@@ -209,108 +271,121 @@ struct RowArgs {
   Consts c;
 };
 
+// Row transforms of length NXB: one wavefront owns whole transforms (lanes
+// exchange through LDS with wave-level ordering) when NXB is a multiple of
+// 64, otherwise the workgroup shares each transform.
 template <typename T, int NX>
-struct RowCtx {
-  using Sh = RowShape<T, NX>;
-  static constexpr int MX = Sh::MX, NXB = Sh::NXB, NT = Sh::NT, NB = Sh::NB;
+struct RowFFT {
+  static constexpr int MX = NX / 2, NXB = 3 * NX / 2, NT = NXB;
+  static constexpr bool WAVE = (NXB % 64) == 0;
+  static constexpr int NW = WAVE ? NXB / 64 : 1;
+  static constexpr int TPW = WAVE ? (6 + NW - 1) / NW : 6;   // transforms per wave
 };
 
-// Synthesise up to NB real row pairs.  For pair q, inputs sa[q]/sb[q] (x-spectral
-// rows, nullptr = zero row) with optional i kx factor; results in ra/rb[q][p].
+// Build the full length-NXB spectrum whose inverse DFT is a + i b for two real
+// rows with half spectra sa (a) and sb (b), k < MX; optional i kx factors.
+template <typename T, int NX>
+__device__ __forceinline__ void fill_pair(cx<T>* z, const cx<T>* sa, bool da, const cx<T>* sb,
+                                          bool db, const T* __restrict__ kxp, int t, int nt) {
+  constexpr int MX = NX / 2, NXB = 3 * NX / 2;
+  for (int k = t; k < MX; k += nt) {
+    cx<T> av = {0, 0}, bv = {0, 0};
+    if (sa) av = sa[k];
+    if (sb) bv = sb[k];
+    const T kx = kxp[k];
+    if (da) av = cx<T>{-kx * av.y, kx * av.x};
+    if (db) bv = cx<T>{-kx * bv.y, kx * bv.x};
+    if (k == 0) {
+      z[0] = cx<T>{av.x, bv.x};      // c2r ignores the imaginary part of the DC term
+    } else {
+      z[k] = cx<T>{av.x - bv.y, av.y + bv.x};
+      z[NXB - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+    }
+  }
+  for (int i = t; i <= MX; i += nt) z[MX + i] = cx<T>{0, 0};
+}
+
+// Synthesise NP (<= 6) real row pairs; this thread's point of the two real
+// rows of pair q comes back in ra[q], rb[q].  Ends with a workgroup barrier
+// after which the caller must not write buf until the next anal_pairs.
 template <typename T, int NX, int NP>
 __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, const bool* da,
-                                            const cx<T>* const* sb, const bool* db,
-                                            T (*ra)[3], T (*rb)[3], const T* __restrict__ kxp,
-                                            const cx<T>* __restrict__ tw, int tid) {
-  using C = RowCtx<T, NX>;
-  constexpr int MX = C::MX, NXB = C::NXB, NT = C::NT, NB = C::NB;
+                                            const cx<T>* const* sb, const bool* db, T* ra, T* rb,
+                                            const T* __restrict__ kxp, const cx<T>* tw, int tid) {
+  using F = RowFFT<T, NX>;
+  constexpr int NXB = F::NXB;
+  if constexpr (F::WAVE) {
+    const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
-  for (int q0 = 0; q0 < NP; q0 += NB) {
-    constexpr int dummy = 0;
-    (void)dummy;
-#pragma unroll
-    for (int qq = 0; qq < NB; ++qq) {
-      const int q = q0 + qq;
-      cx<T>* z = buf + qq * NXB;
-      if (q < NP) {
-        const int k = tid;  // NT == MX
-        cx<T> av = {0, 0}, bv = {0, 0};
-        if (sa[q]) av = sa[q][k];
-        if (sb[q]) bv = sb[q][k];
-        const T kx = kxp[k];
-        if (da[q]) av = cx<T>{-kx * av.y, kx * av.x};
-        if (db[q]) bv = cx<T>{-kx * bv.y, kx * bv.x};
-        if (k == 0) {
-          z[0] = cx<T>{av.x, bv.x};        // c2r ignores the imaginary part of the DC term
-        } else {
-          z[k] = cx<T>{av.x - bv.y, av.y + bv.x};
-          z[NXB - k] = cx<T>{av.x + bv.y, bv.x - av.y};
-        }
-        z[MX + k] = cx<T>{0, 0};
-        if (k == 0) z[2 * MX] = cx<T>{0, 0};
-      }
+    for (int qq = 0; qq < F::TPW; ++qq) {
+      const int q = w * F::TPW + qq;
+      if (q < NP) fill_pair<T, NX>(buf + q * NXB, sa[q], da[q], sb[q], db[q], kxp, lane, 64);
     }
-    __syncthreads();
-    fft_run<T, NXB, true, NB, NT>(buf, tw, tid);
+    wave_sync();
+    if (w * F::TPW < NP) fft_run<T, NXB, true, F::TPW, 64, 0, true>(buf + w * F::TPW * NXB, tw, lane);
+  } else {
 #pragma unroll
-    for (int qq = 0; qq < NB; ++qq) {
-      const int q = q0 + qq;
-      if (q < NP) {
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-          const cx<T> v = buf[qq * NXB + tid + p * NT];
-          ra[q][p] = v.x;
-          rb[q][p] = v.y;
-        }
-      }
-    }
+    for (int q = 0; q < NP; ++q) fill_pair<T, NX>(buf + q * NXB, sa[q], da[q], sb[q], db[q], kxp, tid, F::NT);
     __syncthreads();
+    fft_run<T, NXB, true, NP, F::NT>(buf, tw, tid);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const cx<T> v = buf[q * NXB + tid];
+    ra[q] = v.x;
+    rb[q] = v.y;
   }
 }
 
-// Forward-transform NP real row pairs; returns the first MX coefficients of
-// each member (scaled by 1/NXB) for k = tid.
-template <typename T, int NX, int NP>
-__device__ __forceinline__ void anal_pairs(cx<T>* buf, const T (*xa)[3], const T (*xb)[3],
-                                           cx<T>* oa, cx<T>* ob, const cx<T>* __restrict__ tw,
-                                           int tid) {
-  using C = RowCtx<T, NX>;
-  constexpr int NXB = C::NXB, NT = C::NT, NB = C::NB;
+// Forward-transform NP real row pairs (this thread holds point tid of each)
+// and hand coefficient k < MX of both members, scaled by 1/NXB, to
+// emit(q, k, A, B).
+template <typename T, int NX, int NP, typename Emit>
+__device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb, const cx<T>* tw,
+                                           int tid, Emit emit) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB;
   const T s = T(0.5) / T(NXB);
+  __syncthreads();                       // every thread has read the synthesis results
 #pragma unroll
-  for (int q0 = 0; q0 < NP; q0 += NB) {
+  for (int q = 0; q < NP; ++q) buf[q * NXB + tid] = cx<T>{xa[q], xb[q]};
+  __syncthreads();
+  auto extract = [&](int q, int k) {
+    const cx<T> zk = buf[q * NXB + k];
+    const cx<T> zn = buf[q * NXB + (k == 0 ? 0 : NXB - k)];
+    emit(q, k, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s}, cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
+  };
+  if constexpr (F::WAVE) {
+    const int w = tid >> 6, lane = tid & 63;
+    if (w * F::TPW < NP) {
+      fft_run<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * NXB, tw, lane);
 #pragma unroll
-    for (int qq = 0; qq < NB; ++qq) {
-      const int q = q0 + qq;
-      if (q < NP) {
-#pragma unroll
-        for (int p = 0; p < 3; ++p) buf[qq * NXB + tid + p * NT] = cx<T>{xa[q][p], xb[q][p]};
+      for (int qq = 0; qq < F::TPW; ++qq) {
+        const int q = w * F::TPW + qq;
+        if (q < NP)
+          for (int k = lane; k < MX; k += 64) extract(q, k);
       }
     }
-    __syncthreads();
-    fft_run<T, NXB, false, NB, NT>(buf, tw, tid);
+    // the next synth_pairs only rewrites this wave's own transforms
+  } else {
+    fft_run<T, NXB, false, NP, F::NT>(buf, tw, tid);
+    if (tid < MX) {
 #pragma unroll
-    for (int qq = 0; qq < NB; ++qq) {
-      const int q = q0 + qq;
-      if (q < NP) {
-        const int k = tid;
-        const cx<T> zk = buf[qq * NXB + k];
-        const cx<T> zn = buf[qq * NXB + (k == 0 ? 0 : NXB - k)];
-        oa[q] = cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s};
-        ob[q] = cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s};
-      }
+      for (int q = 0; q < NP; ++q) extract(q, tid);
     }
     __syncthreads();
   }
 }
 
 template <typename T, int NX>
-__global__ void __launch_bounds__(NX / 2) k_rows(RowArgs<T> a) {
-  using C = RowCtx<T, NX>;
-  constexpr int MX = C::MX, NXB = C::NXB, NT = C::NT, NB = C::NB;
+__global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
   constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
-  __shared__ cx<T> buf[NB * NXB];
-  __shared__ double red[7 * NT];
+  __shared__ cx<T> buf[6 * NXB];
+  __shared__ cx<T> tw[NXB];
+  __shared__ double red[RedSize<NT, 7>::N];
   const int tid = threadIdx.x;
   const int y = blockIdx.x;
   const Consts& c = a.c;
@@ -325,13 +400,14 @@ __global__ void __launch_bounds__(NX / 2) k_rows(RowArgs<T> a) {
   };
   auto clampF = [&](int F) { return F <= 0 ? 1 : (F >= nz ? nz - 1 : F); };
   enum { FU = 0, FUY, FV, FVY, FW, FWY, FE, FEY };
+  load_twiddles(tw, a.twNXB, NXB, tid, NT);
 
   const T emin = (T)c.emin;
   const T alk = (T)c.alk, xkmax = (T)c.xkmax;
   const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
   const T c_lo = (T)a.ws->c_lo, c_hi = (T)a.ws->c_hi;
-  const T kx = a.kxp[tid];
+  const bool mn = c.mnout != 0;
   int clampflag = 0;
 
   auto xkk_of = [&](T e) {
@@ -340,15 +416,14 @@ __global__ void __launch_bounds__(NX / 2) k_rows(RowArgs<T> a) {
   };
 
   // window registers (levels iz-1, iz, iz+1 of u, v; faces of w, e)
-  T um[3], u0[3], up[3], vm[3], v0[3], vp[3], wm[3], w0[3], wp[3], em[3], e0[3], ep[3];
-  T wxm[3], wym[3];
+  T um, u0, up, vm, v0, vp, wm, w0, wp, em, e0, ep, wxm, wym;
   {
     // initial window: u,v at z0-1, z0 ; w,e at z0-1, z0 ; wx,wy at face z0-1
     const cx<T>* sa[5];
     const cx<T>* sb[5];
-    bool da[5] = {false, false, false, false, true};
-    bool db[5] = {false, false, false, false, false};
-    T ra[5][3], rb[5][3];
+    const bool da[5] = {false, false, false, false, true};
+    const bool db[5] = {false, false, false, false, false};
+    T ra[5], rb[5];
     const int L0 = z0 - 1, L1 = z0;
     sa[0] = (L0 >= 1 && L0 <= nz) ? row(FU, L0) : nullptr;
     sb[0] = (L0 >= 1 && L0 <= nz) ? row(FV, L0) : nullptr;
@@ -360,28 +435,25 @@ __global__ void __launch_bounds__(NX / 2) k_rows(RowArgs<T> a) {
     sb[3] = row(FE, clampF(L1));
     sa[4] = (L0 >= 1 && L0 <= nz - 1) ? row(FW, L0) : nullptr;
     sb[4] = (L0 >= 1 && L0 <= nz - 1) ? row(FWY, L0) : nullptr;
-    synth_pairs<T, NX, 5>(buf, sa, da, sb, db, ra, rb, a.kxp, a.twNXB, tid);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      um[p] = ra[0][p]; vm[p] = rb[0][p];
-      u0[p] = ra[1][p]; v0[p] = rb[1][p];
-      wm[p] = ra[2][p]; w0[p] = ra[3][p];
-      if (rb[2][p] < emin) clampflag = 1;
-      if (rb[3][p] < emin) clampflag = 1;
-      em[p] = rb[2][p] > emin ? rb[2][p] : emin;
-      e0[p] = rb[3][p] > emin ? rb[3][p] : emin;
-      wxm[p] = ra[4][p]; wym[p] = rb[4][p];
-    }
+    synth_pairs<T, NX, 5>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+    um = ra[0]; vm = rb[0];
+    u0 = ra[1]; v0 = rb[1];
+    wm = ra[2]; w0 = ra[3];
+    if (rb[2] < emin || rb[3] < emin) clampflag = 1;
+    em = rb[2] > emin ? rb[2] : emin;
+    e0 = rb[3] > emin ? rb[3] : emin;
+    wxm = ra[4]; wym = rb[4];
+    __syncthreads();                     // all reads done before waves refill buf
   }
 
   for (int iz = z0; iz < z1; ++iz) {
-    T ux[3], uy[3], vx[3], vy[3], wx0[3], wy0[3], ex[3], ey[3];
+    T ux, uy, vx, vy, wx0, wy0, ex, ey;
     {
       const cx<T>* sa[6];
       const cx<T>* sb[6];
-      bool da[6] = {false, false, true, true, true, true};
-      bool db[6] = {false, false, false, false, false, false};
-      T ra[6][3], rb[6][3];
+      const bool da[6] = {false, false, true, true, true, true};
+      const bool db[6] = {false, false, false, false, false, false};
+      T ra[6], rb[6];
       const int Lp = iz + 1;
       sa[0] = (Lp <= nz) ? row(FU, Lp) : nullptr;
       sb[0] = (Lp <= nz) ? row(FV, Lp) : nullptr;
@@ -397,125 +469,129 @@ __global__ void __launch_bounds__(NX / 2) k_rows(RowArgs<T> a) {
       const int ie = (iz == nz - 1) ? nz - 2 : iz;
       sa[5] = (iz <= nz - 1) ? row(FE, ie) : nullptr;
       sb[5] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
-      synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, a.twNXB, tid);
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        up[p] = ra[0][p]; vp[p] = rb[0][p];
-        wp[p] = ra[1][p];
-        if (rb[1][p] < emin) clampflag = 1;
-        ep[p] = rb[1][p] > emin ? rb[1][p] : emin;
-        ux[p] = ra[2][p]; uy[p] = rb[2][p];
-        vx[p] = ra[3][p]; vy[p] = rb[3][p];
-        wx0[p] = ra[4][p]; wy0[p] = rb[4][p];
-        ex[p] = ra[5][p]; ey[p] = rb[5][p];
-      }
+      synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      up = ra[0]; vp = rb[0];
+      wp = ra[1];
+      if (rb[1] < emin) clampflag = 1;
+      ep = rb[1] > emin ? rb[1] : emin;
+      ux = ra[2]; uy = rb[2];
+      vx = ra[3]; vy = rb[3];
+      wx0 = ra[4]; wy0 = rb[4];
+      ex = ra[5]; ey = rb[5];
     }
     // ---- pointwise tendency (compadv1, complen, compsgs1, compsgs3, sufsim)
-    T fa[6][3], fb[6][3];
-    double s_div = 0, s_uu = 0, s_vv = 0, s_uv = 0, s_ww = 0, s_uw = 0, s_vw = 0;
+    // R = d/dx Fx + d/dy Fy + H per component; the forward pairs are
+    // (Fx_u,H_u) (Fy_u=Fx_v,H_v) (Fx_w,H_w) (Fy_v,Fy_w) (Fx_e,H_e) (Fy_e,0)
+    T fa[6], fb[6];
+    double sums[7];
     const bool top = (iz == nz);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      const T xm = xkk_of(em[p]), x0 = xkk_of(e0[p]), xp = xkk_of(ep[p]);
+    {
+      const T xm = xkk_of(em), x0 = xkk_of(e0), xp = xkk_of(ep);
       const T xs = xm + x0;
-      const T tuu = -xs * ux[p];                                  // compsgs1.f90:144-146
-      const T tvv = -xs * vy[p];
-      const T tuv = -xs * (uy[p] + vx[p]) * T(0.5);
-      const T wz0 = (w0[p] - wm[p]) * dzm;                        // derivs: wz at level iz
-      const T wzp = (wp[p] - w0[p]) * dzm;
+      const T tuu = -xs * ux;                                     // compsgs1.f90:144-146
+      const T tvv = -xs * vy;
+      const T tuv = -xs * (uy + vx) * T(0.5);
+      const T wz0 = (w0 - wm) * dzm;                              // derivs: wz at level iz
+      const T wzp = (wp - w0) * dzm;
       const T tww0 = -xs * wz0;                                   // compsgs1.f90:129,151
       const T twwp = -(x0 + xp) * wzp;
       T tuw0, tvw0, tuwm, tvwm;
       if (top) {                                                  // upperbc.f90:304-305
-        tuw0 = c_hi * (u0[p] + ugal);
-        tvw0 = c_hi * (v0[p] + vgal);
+        tuw0 = c_hi * (u0 + ugal);
+        tvw0 = c_hi * (v0 + vgal);
       } else {                                                    // compsgs1.f90:152-153
-        tuw0 = -x0 * ((up[p] - u0[p]) * dzm + wx0[p]);
-        tvw0 = -x0 * ((vp[p] - v0[p]) * dzm + wy0[p]);
+        tuw0 = -x0 * ((up - u0) * dzm + wx0);
+        tvw0 = -x0 * ((vp - v0) * dzm + wy0);
       }
       if (iz == 1) {                                              // lowerbc.f90:129-130
-        tuwm = -c_lo * (u0[p] + ugal);
-        tvwm = -c_lo * (v0[p] + vgal);
+        tuwm = -c_lo * (u0 + ugal);
+        tvwm = -c_lo * (v0 + vgal);
       } else {
-        tuwm = -xm * ((u0[p] - um[p]) * dzm + wxm[p]);
-        tvwm = -xm * ((v0[p] - vm[p]) * dzm + wym[p]);
+        tuwm = -xm * ((u0 - um) * dzm + wxm);
+        tvwm = -xm * ((v0 - vm) * dzm + wym);
       }
-      // momentum fluxes: R = d/dx Fx + d/dy Fy + H (compadv1.f90:19-51, compsgs3.f90:205-224)
-      const T Fxu = -(u0[p] * u0[p] + tuu);
-      const T Fxy = -(u0[p] * v0[p] + tuv);
-      const T Fyv = -(v0[p] * v0[p] + tvv);
-      const T Hu = T(0.016) - ((up[p] + u0[p]) * w0[p] - (um[p] + u0[p]) * wm[p]) * todzm
-                   - (tuw0 - tuwm) * dzm;
-      const T Hv = -((vp[p] + v0[p]) * w0[p] - (vm[p] + v0[p]) * wm[p]) * todzm
-                   - (tvw0 - tvwm) * dzm;
+      // compadv1.f90:19-51 and compsgs3.f90:205-224
+      fa[0] = -(u0 * u0 + tuu);
+      fb[0] = T(0.016) - ((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
+      fa[1] = -(u0 * v0 + tuv);
+      fb[1] = -((vp + v0) * w0 - (vm + v0) * wm) * todzm - (tvw0 - tvwm) * dzm;
+      fa[3] = -(v0 * v0 + tvv);
       T Fxw = 0, Fyw = 0, Hw = 0, Fxe = 0, Fye = 0, He = 0;
       if (!top) {
-        Fxw = -(T(0.5) * (u0[p] + up[p]) * w0[p] + tuw0);
-        Fyw = -(T(0.5) * (v0[p] + vp[p]) * w0[p] + tvw0);
-        const T wa = T(0.5) * (wp[p] + w0[p]), wb = T(0.5) * (w0[p] + wm[p]);
+        Fxw = -(T(0.5) * (u0 + up) * w0 + tuw0);
+        Fyw = -(T(0.5) * (v0 + vp) * w0 + tvw0);
+        const T wa = T(0.5) * (wp + w0), wb = T(0.5) * (w0 + wm);
         Hw = -(wa * wa - wb * wb) * dzm - (twwp - tww0) * dzm;
         // SGS TKE: advection (compadv1.f90:59-64), diffusion and dissipation
         // (compsgs3.f90:185-200)
-        Fxe = -T(0.5) * e0[p] * (u0[p] + up[p]) + T(2) * x0 * ex[p];
-        Fye = -T(0.5) * e0[p] * (v0[p] + vp[p]) + T(2) * x0 * ey[p];
-        He = -(wp[p] * ep[p] - wm[p] * em[p]) * todzm
-             + ((xp + x0) * (ep[p] - e0[p]) - (x0 + xm) * (e0[p] - em[p])) * dzm2
-             - ceps * e0[p] * sqrt(e0[p]);
+        Fxe = -T(0.5) * e0 * (u0 + up) + T(2) * x0 * ex;
+        Fye = -T(0.5) * e0 * (v0 + vp) + T(2) * x0 * ey;
+        He = -(wp * ep - wm * em) * todzm
+             + ((xp + x0) * (ep - e0) - (x0 + xm) * (e0 - em)) * dzm2 - ceps * e0 * sqrt(e0);
       }
-      fa[0][p] = Fxu; fb[0][p] = Hu;
-      fa[1][p] = Fxy; fb[1][p] = Hv;
-      fa[2][p] = Fyv; fb[2][p] = Fxw;
-      fa[3][p] = Hw;  fb[3][p] = Fyw;
-      fa[4][p] = Fxe; fb[4][p] = He;
-      fa[5][p] = Fye; fb[5][p] = T(0);
-      const double dv = (double)(ux[p] + vy[p] + wz0);            // comp1.f90:145
-      s_div += dv * dv;
-      if (c.mnout) {
-        s_uu += (double)tuu; s_vv += (double)tvv; s_uv += (double)tuv;
-        s_ww += (double)tww0; s_uw += (double)tuw0; s_vw += (double)tvw0;
+      fa[2] = Fxw; fb[2] = Hw;
+      fb[3] = Fyw;
+      fa[4] = Fxe; fb[4] = He;
+      fa[5] = Fye; fb[5] = T(0);
+      const double dv = (double)(ux + vy + wz0);                  // comp1.f90:145
+      sums[0] = dv * dv;
+      sums[1] = (double)tuu; sums[2] = (double)tvv; sums[3] = (double)tuv;
+      sums[4] = (double)tww0; sums[5] = (double)tuw0; sums[6] = (double)tvw0;
+    }
+    // partial sums of divz and (when means are due) the SGS stress means
+    if constexpr (F::WAVE) {
+      const int nq = mn ? 7 : 1;
+      for (int q = 0; q < nq; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sums[q] += __shfl_xor(sums[q], off, 64);
+      }
+      if ((tid & 63) == 0)
+        for (int q = 0; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
+    } else {
+      if (mn) {
+        block_sum<NT, 7>(sums, red, tid);
+        if (tid < 7) a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sums[0];
+      } else {
+        double d1[1] = {sums[0]};
+        block_sum<NT, 1>(d1, red, tid);
+        if (tid == 0) a.part[(size_t)iz * NYB + y] = d1[0];
       }
     }
-    cx<T> oa[6], ob[6];
-    anal_pairs<T, NX, 6>(buf, fa, fb, oa, ob, a.twNXB, tid);
     {
-      const cx<T> ikx = {0, kx};
-      const size_t o = (size_t)iz * plane + rowoff + tid;
       cx<T>* A = a.A;
-      // G = i kx Fx + H ; Fy kept for the y pass
-      A[0 * a.astride + o] = cadd(cmul(ikx, oa[0]), ob[0]);        // Gu
-      A[1 * a.astride + o] = oa[1];                                // Fyu = Fxy
-      A[2 * a.astride + o] = cadd(cmul(ikx, oa[1]), ob[1]);        // Gv
-      A[3 * a.astride + o] = oa[2];                                // Fyv
-      if (!top) {
-        A[4 * a.astride + o] = cadd(cmul(ikx, ob[2]), oa[3]);      // Gw
-        A[5 * a.astride + o] = ob[3];                              // Fyw
-        A[6 * a.astride + o] = cadd(cmul(ikx, oa[4]), ob[4]);      // Ge
-        A[7 * a.astride + o] = oa[5];                              // Fye
+      const size_t o = (size_t)iz * plane + rowoff;
+      const size_t as = a.astride;
+      const T* kxp = a.kxp;
+      auto emit = [&](int q, int k, cx<T> fA, cx<T> fB) {
+        const cx<T> ikx = {0, kxp[k]};
+        switch (q) {
+          case 0: A[0 * as + o + k] = cadd(cmul(ikx, fA), fB); break;          // Gu
+          case 1: A[1 * as + o + k] = fA;                                       // Fyu
+                  A[2 * as + o + k] = cadd(cmul(ikx, fA), fB); break;          // Gv
+          case 2: if (!top) A[4 * as + o + k] = cadd(cmul(ikx, fA), fB); break; // Gw
+          case 3: A[3 * as + o + k] = fA;                                       // Fyv
+                  if (!top) A[5 * as + o + k] = fB; break;                     // Fyw
+          case 4: if (!top) A[6 * as + o + k] = cadd(cmul(ikx, fA), fB); break; // Ge
+          default: if (!top) A[7 * as + o + k] = fA; break;                    // Fye
+        }
+      };
+      anal_pairs<T, NX, 6>(buf, fa, fb, tw, tid, emit);
+    }
+    if constexpr (F::WAVE) {
+      // wave partials were published by the barriers inside anal_pairs
+      const int nq = mn ? 7 : 1;
+      if (tid < nq) {
+        double sq = 0;
+        for (int w = 0; w < F::NW; ++w) sq += red[tid * F::NW + w];
+        a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sq;
       }
     }
-    // block sums of divz and the SGS stress means
-    red[0 * NT + tid] = s_div;
-    red[1 * NT + tid] = s_uu; red[2 * NT + tid] = s_vv; red[3 * NT + tid] = s_uv;
-    red[4 * NT + tid] = s_ww; red[5 * NT + tid] = s_uw; red[6 * NT + tid] = s_vw;
-    __syncthreads();
-    for (int h = NT / 2; h > 0; h >>= 1) {
-      if (tid < h) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) red[q * NT + tid] += red[q * NT + tid + h];
-      }
-      __syncthreads();
-    }
-    if (tid < 7) a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = red[tid * NT];
-    __syncthreads();
     // slide the window
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      um[p] = u0[p]; u0[p] = up[p];
-      vm[p] = v0[p]; v0[p] = vp[p];
-      wm[p] = w0[p]; w0[p] = wp[p];
-      em[p] = e0[p]; e0[p] = ep[p];
-      wxm[p] = wx0[p]; wym[p] = wy0[p];
-    }
+    um = u0; u0 = up;
+    vm = v0; v0 = vp;
+    wm = w0; w0 = wp;
+    em = e0; e0 = ep;
+    wxm = wx0; wym = wy0;
   }
   if (clampflag) atomicOr(a.flags, FLAG_ECLAMP);
 }
@@ -546,12 +622,16 @@ __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
   const int tid = threadIdx.x;
   __shared__ cx<T> spec[TK * NY];
   __shared__ cx<T> buf[2 * TK * NYB];
+  __shared__ cx<T> twb[NYB];
+  __shared__ cx<T> tws[NY];
   const size_t splane = (size_t)NY * MX;
   if (cmp >= 2 && z == a.nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
     cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
     for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
     return;
   }
+  load_twiddles(twb, a.twNYB, NYB, tid, NT);
+  if (cmp == 3) load_twiddles(tws, a.twNY, NY, tid, NT);
   const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + k0;
   const cx<T>* fy = g + a.astride;
   for (int i = tid; i < 2 * TK * NYB; i += NT) {
@@ -562,7 +642,8 @@ __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
     buf[(d * TK + c) * NYB + n] = (d ? fy : g)[(size_t)n * MX + c];
   }
   __syncthreads();
-  fft_run<T, NYB, false, 2 * TK, NT>(buf, a.twNYB, tid);
+  fft_waves<T, NYB, false, 2 * TK, NT / 64>(buf, twb, tid);
+  __syncthreads();
   const T s = T(1) / T(NYB);
   for (int i = tid; i < TK * NY; i += NT) {
     const int c = i % TK, j = i / TK;
@@ -583,7 +664,8 @@ __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
   }
   if (cmp < 3) return;
   __syncthreads();
-  fft_run<T, NY, true, TK, NT>(spec, a.twNY, tid);
+  fft_waves<T, NY, true, TK, NT / 64>(spec, tws, tid);
+  __syncthreads();
   cx<T>* o = a.Tre + (size_t)z * splane + k0;
   for (int i = tid; i < TK * NY; i += NT) {
     const int c = i % TK, y = i / TK;
@@ -593,7 +675,7 @@ __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // k_press: substep + Poisson RHS + tridiagonal z-solve (FP64) + projection and
-// RK update.  One thread per band mode (kx, j).  grid ceil(NY*MX/256)
+// RK update.  One thread per band mode (kx, j).  grid ceil(NY*MX/64)
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PressArgs {
@@ -620,7 +702,7 @@ __device__ __forceinline__ void st2(cx<T>* p, cx<double> v) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_press(PressArgs<T> a) {
+__global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   const int modes = a.mx * a.ny;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= modes) return;
@@ -639,6 +721,7 @@ __global__ void __launch_bounds__(256) k_press(PressArgs<T> a) {
   for (int z = 1; z <= nz; ++z) {
     const size_t o = (size_t)z * pl + m;
     cx<double> u = ld2(a.U[0] + o), v = ld2(a.U[1] + o), w = ld2(a.U[2] + o);
+    const cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
     if (sub) {                                   // comp1.f90:126-133
       u = csub(u, cscale(ld2(a.Ro[0] + o), a.dtab2));
       v = csub(v, cscale(ld2(a.Ro[1] + o), a.dtab2));
@@ -647,7 +730,6 @@ __global__ void __launch_bounds__(256) k_press(PressArgs<T> a) {
       st2(a.U[1] + o, v);
       st2(a.U[2] + o, w);
     }
-    const cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
     const cx<double> fx = cadd(ru, cscale(u, a.dtabm));
     const cx<double> fy = cadd(rv, cscale(v, a.dtabm));
     cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
@@ -683,10 +765,10 @@ __global__ void __launch_bounds__(256) k_press(PressArgs<T> a) {
     const size_t o = (size_t)z * pl + m;
     const double d = a.sd[o];
     const cx<double> r = a.sr[o];
-    cx<double> p = (z == nz) ? cscale(r, d) : cscale(csub(r, cscale(p_next, up_c)), d);
-    if (zero_mode) p = cx<double>{0, 0};         // press.F90:151-152
     const cx<double> u = ld2(a.U[0] + o), v = ld2(a.U[1] + o), w = ld2(a.U[2] + o);
     cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
+    cx<double> p = (z == nz) ? cscale(r, d) : cscale(csub(r, cscale(p_next, up_c)), d);
+    if (zero_mode) p = cx<double>{0, 0};         // press.F90:151-152
     ru = csub(ru, cmuli(cscale(p, kxp)));        // comp2.f90:241-242
     rv = csub(rv, cmuli(cscale(p, kyp)));        // comp2.f90:246-247
     cx<double> wn = {0, 0};
@@ -732,6 +814,7 @@ template <typename T, int NX>
 __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
   constexpr int MX = NX / 2, NT = MX, NY = NX;
   __shared__ cx<T> buf[NX];
+  __shared__ cx<T> tw[NX];
   const int tid = threadIdx.x;
   const int y0 = blockIdx.x, y1 = blockIdx.x + NY / 2;
   const int z = blockIdx.y + 1;
@@ -746,6 +829,8 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
     }
     return;
   }
+  if (a.mode != 0 && z >= nz) return;
+  load_twiddles(tw, a.twNX, NX, tid, NT);
   T ea[2], eb[2];
   if (a.mode == 0) {
     // synthesise re on rows y0 (real part) and y1 (imaginary part)
@@ -761,7 +846,7 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
       buf[NX - k] = cx<T>{av.x + bv.y, bv.x - av.y};
     }
     __syncthreads();
-    fft_run<T, NX, true, 1, NT>(buf, a.twNX, tid);
+    fft_run<T, NX, true, 1, NT>(buf, tw, tid);
     int bad = 0;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
@@ -784,7 +869,7 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
       e1 = e1 + (T)a.dtab1 * r.y;
       e0 = e0 > (T)a.emin ? e0 : (T)a.emin;
       e1 = e1 > (T)a.emin ? e1 : (T)a.emin;
-      if (!(e0 == e0) || !(e1 == e1) || !isfinite(e0) || !isfinite(e1)) bad = 1;
+      if (!isfinite(e0) || !isfinite(e1)) bad = 1;
       *pe0 = e0; *pe1 = e1;
       *pr0 = r.x; *pr1 = r.y;
       ea[p] = e0; eb[p] = e1;
@@ -792,7 +877,6 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
     if (bad) atomicOr(a.flags, FLAG_NONFINITE);
     __syncthreads();
   } else {
-    if (z >= nz) return;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       ea[p] = e[(size_t)y0 * NX + tid + p * NT];
@@ -803,7 +887,7 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
 #pragma unroll
   for (int p = 0; p < 2; ++p) buf[tid + p * NT] = cx<T>{ea[p], eb[p]};
   __syncthreads();
-  fft_run<T, NX, false, 1, NT>(buf, a.twNX, tid);
+  fft_run<T, NX, false, 1, NT>(buf, tw, tid);
   const T s = T(0.5) / T(NX);
   const int k = tid;
   const cx<T> zk = buf[k];
@@ -820,10 +904,12 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
 // physical double rows -> x-spectral rows (x 1/NX, Nyquist dropped); grid (NY/2, nl)
 template <typename T, int NX>
 __global__ void __launch_bounds__(NX / 2) k_rows_fwd(const double* __restrict__ in, cx<T>* out,
-                                                      const cx<T>* tw, int l0) {
+                                                      const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = MX, NY = NX;
   __shared__ cx<T> buf[NX];
+  __shared__ cx<T> tw[NX];
   const int tid = threadIdx.x;
+  load_twiddles(tw, twg, NX, tid, NT);
   const int y0 = blockIdx.x, y1 = y0 + NY / 2;
   const int L = l0 + blockIdx.y;
   const double* p = in + (size_t)blockIdx.y * NY * NX;
@@ -845,10 +931,12 @@ __global__ void __launch_bounds__(NX / 2) k_rows_fwd(const double* __restrict__ 
 // x-spectral rows -> physical double rows (unnormalised c2r); grid (NY/2, nl)
 template <typename T, int NX>
 __global__ void __launch_bounds__(NX / 2) k_rows_inv(const cx<T>* __restrict__ in, double* out,
-                                                      const cx<T>* tw, int l0) {
+                                                      const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = MX, NY = NX;
   __shared__ cx<T> buf[NX];
+  __shared__ cx<T> tw[NX];
   const int tid = threadIdx.x;
+  load_twiddles(tw, twg, NX, tid, NT);
   const int y0 = blockIdx.x, y1 = y0 + NY / 2;
   const int L = l0 + blockIdx.y;
   const cx<T>* s0 = in + (size_t)L * NY * MX + (size_t)y0 * MX;
@@ -876,11 +964,13 @@ __global__ void __launch_bounds__(NX / 2) k_rows_inv(const cx<T>* __restrict__ i
 // x-spectral small rows <-> band spectrum along y; grid (MX/TK, nl)
 template <typename T, int NX, int NY, bool INV>
 __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in, cx<T>* out,
-                                                     const cx<T>* tw, int l0) {
+                                                     const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = 256;
   constexpr int TK = ColShape<T, NX, NY>::TK;
   __shared__ cx<T> buf[TK * NY];
+  __shared__ cx<T> tw[NY];
   const int tid = threadIdx.x;
+  load_twiddles(tw, twg, NY, tid, NT);
   const int k0 = blockIdx.x * TK;
   const int L = l0 + blockIdx.y;
   const cx<T>* src = in + (size_t)L * NY * MX + k0;
@@ -891,7 +981,8 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
     buf[c * NY + j] = v;
   }
   __syncthreads();
-  fft_run<T, NY, INV, TK, NT>(buf, tw, tid);
+  fft_waves<T, NY, INV, TK, NT / 64>(buf, tw, tid);
+  __syncthreads();
   cx<T>* dst = out + (size_t)L * NY * MX + k0;
   const T s = INV ? T(1) : T(1) / T(NY);
   for (int i = tid; i < TK * NY; i += NT) {

@@ -306,7 +306,7 @@ struct EngineT : Engine {
       a.dtab2 = c.dtab2;
       a.dtabm = c.dtabm;
       const int modes = MX * NY;
-      hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 255) / 256), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 63) / 64), dim3(64), 0, s, a);
     }
     mark();
     {
