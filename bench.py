@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=0,
                     help="levels of the CPU-baseline sample (default: the full nz)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one device copy of 256 MiB (counter calibration under rocprofv3)")
     return ap.parse_args()
 
 
@@ -121,6 +123,13 @@ def main():
                                    "(the multi-GPU z-slab design is described, not built)"}))
         sys.exit(2)
     pts = args.nx * args.nx * args.nz
+    if args.copy_probe:
+        import torch
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] copy probe: 256 MiB read + 256 MiB written", file=sys.stderr, flush=True)
     print(f"[bench] fp32 path {args.nx}x{args.nx}x{args.nz}", file=sys.stderr, flush=True)
     r32 = run_gpu(args, "fp32")
     value = pts * args.steps / (r32["ms"] * 1e-3)

@@ -85,6 +85,12 @@ constexpr int span_before(int n, int s) {
   return p;
 }
 
+// LDS layout of one transform: a pad element after every 8 so that the
+// stride-R writes of a first stage spread over the banks.  Element i of a
+// transform sits at pidx(i); transforms are padlen(N) elements apart.
+constexpr int padlen(int n) { return n + n / 8; }
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void butterfly(cx<T>* v) {
   if constexpr (R == 2) {
@@ -193,9 +199,9 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
     if ((TOT % NT) == 0 || idx < TOT) {
       const int t = idx / NBF;
       const int j = idx - t * NBF;
-      const cx<T>* src = a + t * N + j;
+      const cx<T>* src = a + t * padlen(N);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[r * NBF];
+      for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * NBF)];
       if constexpr (NS > 1) {
         const int k = j % NS;
 #pragma unroll
@@ -215,9 +221,10 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
       const int t = idx / NBF;
       const int j = idx - t * NBF;
       const int k = j % NS;
-      cx<T>* dst = a + t * N + (j / NS) * NS * R + k;
+      cx<T>* dst = a + t * padlen(N);
+      const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+      for (int r = 0; r < R; ++r) dst[pidx(base + r * NS)] = v[q][r];
     }
   }
   fft_sync<WAVE>();
@@ -231,7 +238,7 @@ __device__ __forceinline__ void fft_run(cx<T>* a, const cx<T>* __restrict__ tw, 
   }
 }
 
-// NBT transforms spread over the NW wavefronts of a workgroup, each wave
+// NBT transforms (padded layout, see padlen) spread over the NW wavefronts of a workgroup, each wave
 // running its share with wave-level ordering only.  The caller synchronises
 // the workgroup before (inputs written) and after (outputs read by others).
 template <typename T, int N, bool INV, int NBT, int NW>
@@ -239,9 +246,9 @@ __device__ __forceinline__ void fft_waves(cx<T>* a, const cx<T>* __restrict__ tw
   const int w = tid >> 6, lane = tid & 63;
   if constexpr (NBT % NW == 0) {
     constexpr int PW = NBT / NW;
-    fft_run<T, N, INV, PW, 64, 0, true>(a + w * PW * N, tw, lane);
+    fft_run<T, N, INV, PW, 64, 0, true>(a + w * PW * padlen(N), tw, lane);
   } else {
-    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true>(a + t * N, tw, lane);
+    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true>(a + t * padlen(N), tw, lane);
   }
 }
 

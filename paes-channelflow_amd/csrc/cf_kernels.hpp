@@ -190,38 +190,42 @@ __global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
   if (f >= 2 && z >= a.nz) return;       // w, e live on faces 1..nz-1
   const int k0 = blockIdx.x * TK;
   const int tid = threadIdx.x;
-  __shared__ cx<T> spec[TK * NY];
-  __shared__ cx<T> buf[2 * TK * NYB];
+  __shared__ cx<T> spec[TK * padlen(NY)];
+  __shared__ cx<T> buf[2 * TK * padlen(NYB)];
   __shared__ cx<T> twb[NYB];
   __shared__ cx<T> tws[NY];
   load_twiddles(twb, a.twNYB, NYB, tid, NT);
 
   if (f < 3) {
     const cx<T>* src = a.S[f] + (size_t)z * NY * MX + k0;
+#pragma unroll
     for (int i = tid; i < TK * NY; i += NT) {
       const int c = i % TK, j = i / TK;
-      spec[c * NY + j] = src[(size_t)j * MX + c];
+      spec[c * padlen(NY) + pidx(j)] = src[(size_t)j * MX + c];
     }
     __syncthreads();
   } else {
     load_twiddles(tws, a.twNY, NY, tid, NT);
     const cx<T>* src = a.Ex + (size_t)z * NY * MX + k0;
+#pragma unroll
     for (int i = tid; i < TK * NY; i += NT) {
       const int c = i % TK, y = i / TK;
-      spec[c * NY + y] = src[(size_t)y * MX + c];
+      spec[c * padlen(NY) + pidx(y)] = src[(size_t)y * MX + c];
     }
     __syncthreads();
     fft_waves<T, NY, false, TK, NT / 64>(spec, tws, tid);
     __syncthreads();
     const T s = T(1) / T(NY);
+#pragma unroll
     for (int i = tid; i < TK * NY; i += NT) {
-      const int j = i % NY;
-      cx<T> v = spec[i];
-      spec[i] = (j == NY / 2) ? cx<T>{0, 0} : cscale(v, s);
+      const int c = i / NY, j = i % NY;
+      cx<T>& v = spec[c * padlen(NY) + pidx(j)];
+      v = (j == NY / 2) ? cx<T>{0, 0} : cscale(v, s);
     }
     __syncthreads();
   }
   // zero-pad the band into the 3/2 grid; second half of the batch holds i ky S
+#pragma unroll
   for (int i = tid; i < 2 * TK * NYB; i += NT) {
     const int t = i / NYB, n = i - t * NYB;
     const int c = t % TK;
@@ -230,25 +234,26 @@ __global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
     else if (n > NYB - NY / 2) j = n - NYB + NY;
     cx<T> v = {0, 0};
     if (j >= 0) {
-      v = spec[c * NY + j];
+      v = spec[c * padlen(NY) + pidx(j)];
       if (t >= TK) {
         const T k = a.kyp[j];
         v = cx<T>{-k * v.y, k * v.x};
       }
     }
-    buf[i] = v;
+    buf[t * padlen(NYB) + pidx(n)] = v;
   }
   __syncthreads();
   fft_waves<T, NYB, true, 2 * TK, NT / 64>(buf, twb, tid);
   __syncthreads();
   cx<T>* o0 = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
   cx<T>* o1 = o0 + a.bstride;
+#pragma unroll
   for (int i = tid; i < 2 * TK * NYB; i += NT) {
     const int c = i % TK;
     const int r = i / TK;
     const int n = r % NYB;
     const int d = r / NYB;
-    (d ? o1 : o0)[(size_t)n * MX + c] = buf[(d * TK + c) * NYB + n];
+    (d ? o1 : o0)[(size_t)n * MX + c] = buf[(d * TK + c) * padlen(NYB) + pidx(n)];
   }
 }
 
@@ -298,11 +303,11 @@ __device__ __forceinline__ void fill_pair(cx<T>* z, const cx<T>* sa, bool da, co
     if (k == 0) {
       z[0] = cx<T>{av.x, bv.x};      // c2r ignores the imaginary part of the DC term
     } else {
-      z[k] = cx<T>{av.x - bv.y, av.y + bv.x};
-      z[NXB - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+      z[pidx(k)] = cx<T>{av.x - bv.y, av.y + bv.x};
+      z[pidx(NXB - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
     }
   }
-  for (int i = t; i <= MX; i += nt) z[MX + i] = cx<T>{0, 0};
+  for (int i = t; i <= MX; i += nt) z[pidx(MX + i)] = cx<T>{0, 0};
 }
 
 // Synthesise NP (<= 6) real row pairs; this thread's point of the two real
@@ -319,20 +324,20 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
 #pragma unroll
     for (int qq = 0; qq < F::TPW; ++qq) {
       const int q = w * F::TPW + qq;
-      if (q < NP) fill_pair<T, NX>(buf + q * NXB, sa[q], da[q], sb[q], db[q], kxp, lane, 64);
+      if (q < NP) fill_pair<T, NX>(buf + q * padlen(NXB), sa[q], da[q], sb[q], db[q], kxp, lane, 64);
     }
     wave_sync();
-    if (w * F::TPW < NP) fft_run<T, NXB, true, F::TPW, 64, 0, true>(buf + w * F::TPW * NXB, tw, lane);
+    if (w * F::TPW < NP) fft_run<T, NXB, true, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw, lane);
   } else {
 #pragma unroll
-    for (int q = 0; q < NP; ++q) fill_pair<T, NX>(buf + q * NXB, sa[q], da[q], sb[q], db[q], kxp, tid, F::NT);
+    for (int q = 0; q < NP; ++q) fill_pair<T, NX>(buf + q * padlen(NXB), sa[q], da[q], sb[q], db[q], kxp, tid, F::NT);
     __syncthreads();
     fft_run<T, NXB, true, NP, F::NT>(buf, tw, tid);
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
-    const cx<T> v = buf[q * NXB + tid];
+    const cx<T> v = buf[q * padlen(NXB) + pidx(tid)];
     ra[q] = v.x;
     rb[q] = v.y;
   }
@@ -349,17 +354,17 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
   const T s = T(0.5) / T(NXB);
   __syncthreads();                       // every thread has read the synthesis results
 #pragma unroll
-  for (int q = 0; q < NP; ++q) buf[q * NXB + tid] = cx<T>{xa[q], xb[q]};
+  for (int q = 0; q < NP; ++q) buf[q * padlen(NXB) + pidx(tid)] = cx<T>{xa[q], xb[q]};
   __syncthreads();
   auto extract = [&](int q, int k) {
-    const cx<T> zk = buf[q * NXB + k];
-    const cx<T> zn = buf[q * NXB + (k == 0 ? 0 : NXB - k)];
+    const cx<T> zk = buf[q * padlen(NXB) + pidx(k)];
+    const cx<T> zn = buf[q * padlen(NXB) + pidx(k == 0 ? 0 : NXB - k)];
     emit(q, k, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s}, cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
   };
   if constexpr (F::WAVE) {
     const int w = tid >> 6, lane = tid & 63;
     if (w * F::TPW < NP) {
-      fft_run<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * NXB, tw, lane);
+      fft_run<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw, lane);
 #pragma unroll
       for (int qq = 0; qq < F::TPW; ++qq) {
         const int q = w * F::TPW + qq;
@@ -383,7 +388,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   using F = RowFFT<T, NX>;
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
   constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
-  __shared__ cx<T> buf[6 * NXB];
+  __shared__ cx<T> buf[6 * padlen(NXB)];
   __shared__ cx<T> tw[NXB];
   __shared__ double red[RedSize<NT, 7>::N];
   const int tid = threadIdx.x;
@@ -446,29 +451,36 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     __syncthreads();                     // all reads done before waves refill buf
   }
 
+  // the six synthesis pairs of level iz: u,v at iz+1; w,e at face iz+1;
+  // (du/dx, du/dy), (dv/dx, dv/dy), (dw/dx, dw/dy) at iz; (de/dx, de/dy) at
+  // face iz -- except at iz = nnz-1, where ex, ey are not refreshed
+  // (derivs2.f90:279) and face nnz-2 is reused.  The first member of pairs
+  // 2..5 takes an i kx factor.
+  auto level_srcs = [&](int iz, const cx<T>** sa, const cx<T>** sb) {
+    const int Lp = iz + 1;
+    sa[0] = (Lp <= nz) ? row(FU, Lp) : nullptr;
+    sb[0] = (Lp <= nz) ? row(FV, Lp) : nullptr;
+    sa[1] = (Lp <= nz - 1) ? row(FW, Lp) : nullptr;
+    sb[1] = row(FE, clampF(Lp));
+    sa[2] = row(FU, iz);
+    sb[2] = row(FUY, iz);
+    sa[3] = row(FV, iz);
+    sb[3] = row(FVY, iz);
+    sa[4] = (iz <= nz - 1) ? row(FW, iz) : nullptr;
+    sb[4] = (iz <= nz - 1) ? row(FWY, iz) : nullptr;
+    const int ie = (iz == nz - 1) ? nz - 2 : iz;
+    sa[5] = (iz <= nz - 1) ? row(FE, ie) : nullptr;
+    sb[5] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
+  };
   for (int iz = z0; iz < z1; ++iz) {
     T ux, uy, vx, vy, wx0, wy0, ex, ey;
     {
+      T ra[6], rb[6];
       const cx<T>* sa[6];
       const cx<T>* sb[6];
       const bool da[6] = {false, false, true, true, true, true};
       const bool db[6] = {false, false, false, false, false, false};
-      T ra[6], rb[6];
-      const int Lp = iz + 1;
-      sa[0] = (Lp <= nz) ? row(FU, Lp) : nullptr;
-      sb[0] = (Lp <= nz) ? row(FV, Lp) : nullptr;
-      sa[1] = (Lp <= nz - 1) ? row(FW, Lp) : nullptr;
-      sb[1] = row(FE, clampF(Lp));
-      sa[2] = row(FU, iz);
-      sb[2] = row(FUY, iz);
-      sa[3] = row(FV, iz);
-      sb[3] = row(FVY, iz);
-      sa[4] = (iz <= nz - 1) ? row(FW, iz) : nullptr;
-      sb[4] = (iz <= nz - 1) ? row(FWY, iz) : nullptr;
-      // ex, ey are not refreshed at iz = nnz-1 (derivs2.f90:279): reuse face nnz-2
-      const int ie = (iz == nz - 1) ? nz - 2 : iz;
-      sa[5] = (iz <= nz - 1) ? row(FE, ie) : nullptr;
-      sb[5] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
+      level_srcs(iz, sa, sb);
       synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
       up = ra[0]; vp = rb[0];
       wp = ra[1];
@@ -612,64 +624,105 @@ struct AnalArgs {
   int nz;
 };
 
+// Load the flux columns of tile t (see k_col_anal) into this thread's registers.
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void anal_load_tile(const AnalArgs<T>& a, int t,
+                                               cx<T> (&pre)[2 * ColShape<T, NX, NY>::TK * (3 * NY / 2) / 256],
+                                               int tid) {
+  using Sh = ColShape<T, NX, NY>;
+  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
+  constexpr int NKT = MX / TK;
+  constexpr int LPT = 2 * TK * NYB / NT;
+  const int kt = t % NKT, rest = t / NKT;
+  const int z = rest % a.nz + 1, cmp = rest / a.nz;
+  if (cmp >= 2 && z == a.nz) return;
+  const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + kt * TK;
+#pragma unroll
+  for (int i = 0; i < LPT; ++i) {
+    const int idx = tid + i * NT;
+    const int c = idx % TK, r = idx / TK;
+    const int n = r % NYB, d = r / NYB;
+    pre[i] = g[(size_t)d * a.astride + (size_t)n * MX + c];
+  }
+}
+
+// Persistent: each workgroup walks tiles (kx tile fastest, then level, then
+// component), holding the next tile's columns in registers while the current
+// tile's transforms run.
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
-  const int cmp = blockIdx.z;
-  const int z = blockIdx.y + 1;
-  const int k0 = blockIdx.x * TK;
+  constexpr int NKT = MX / TK;
+  constexpr int LPT = 2 * TK * NYB / NT;       // prefetched elements per thread
+  static_assert((2 * TK * NYB) % NT == 0, "tile must divide over the workgroup");
+  constexpr int PLB = padlen(NYB), PLS = padlen(NY);
   const int tid = threadIdx.x;
-  __shared__ cx<T> spec[TK * NY];
-  __shared__ cx<T> buf[2 * TK * NYB];
+  __shared__ cx<T> spec[TK * PLS];
+  __shared__ cx<T> buf[2 * TK * PLB];
   __shared__ cx<T> twb[NYB];
   __shared__ cx<T> tws[NY];
   const size_t splane = (size_t)NY * MX;
-  if (cmp >= 2 && z == a.nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
-    cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
-    for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
-    return;
-  }
+  const int nz = a.nz;
+  const int ntiles = NKT * nz * 4;
   load_twiddles(twb, a.twNYB, NYB, tid, NT);
-  if (cmp == 3) load_twiddles(tws, a.twNY, NY, tid, NT);
-  const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + k0;
-  const cx<T>* fy = g + a.astride;
-  for (int i = tid; i < 2 * TK * NYB; i += NT) {
-    const int c = i % TK;
-    const int r = i / TK;
-    const int n = r % NYB;
-    const int d = r / NYB;
-    buf[(d * TK + c) * NYB + n] = (d ? fy : g)[(size_t)n * MX + c];
-  }
-  __syncthreads();
-  fft_waves<T, NYB, false, 2 * TK, NT / 64>(buf, twb, tid);
-  __syncthreads();
-  const T s = T(1) / T(NYB);
-  for (int i = tid; i < TK * NY; i += NT) {
-    const int c = i % TK, j = i / TK;
-    cx<T> v = {0, 0};
-    if (j != NY / 2) {
-      const int n = (j < NY / 2) ? j : j - NY + NYB;
-      const cx<T> gv = buf[c * NYB + n];
-      const cx<T> fv = buf[(TK + c) * NYB + n];
-      const T k = a.kyp[j];
-      v = cx<T>{(gv.x - k * fv.y) * s, (gv.y + k * fv.x) * s};
-      if (cmp == 2 && j == 0 && k0 + c == 0) v = cx<T>{0, 0};     // rw mean removal (comp1.f90:172)
+  load_twiddles(tws, a.twNY, NY, tid, NT);
+  cx<T> pre[LPT];
+  int t = blockIdx.x;
+  if (t < ntiles) anal_load_tile<T, NX, NY>(a, t, pre, tid);
+  for (; t < ntiles; t += gridDim.x) {
+    const int kt = t % NKT, rest = t / NKT;
+    const int z = rest % nz + 1, cmp = rest / nz;
+    const int k0 = kt * TK;
+    const int tn = t + gridDim.x;
+    if (cmp >= 2 && z == nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
+      cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
+      for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
+      if (tn < ntiles) anal_load_tile<T, NX, NY>(a, tn, pre, tid);
+      continue;
     }
-    if (cmp < 3) {
-      a.R[cmp][(size_t)z * splane + (size_t)j * MX + k0 + c] = v;
-    } else {
-      spec[c * NY + j] = v;
+    __syncthreads();                           // the previous tile is out of LDS
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int idx = tid + i * NT;
+      const int c = idx % TK, r = idx / TK;
+      const int n = r % NYB, d = r / NYB;
+      buf[(d * TK + c) * PLB + pidx(n)] = pre[i];
     }
-  }
-  if (cmp < 3) return;
-  __syncthreads();
-  fft_waves<T, NY, true, TK, NT / 64>(spec, tws, tid);
-  __syncthreads();
-  cx<T>* o = a.Tre + (size_t)z * splane + k0;
-  for (int i = tid; i < TK * NY; i += NT) {
-    const int c = i % TK, y = i / TK;
-    o[(size_t)y * MX + c] = spec[c * NY + y];
+    if (tn < ntiles) anal_load_tile<T, NX, NY>(a, tn, pre, tid);   // in flight during the transforms
+    __syncthreads();
+    fft_waves<T, NYB, false, 2 * TK, NT / 64>(buf, twb, tid);
+    __syncthreads();
+    const T s = T(1) / T(NYB);
+#pragma unroll
+    for (int i = tid; i < TK * NY; i += NT) {
+      const int c = i % TK, j = i / TK;
+      cx<T> v = {0, 0};
+      if (j != NY / 2) {
+        const int n = (j < NY / 2) ? j : j - NY + NYB;
+        const cx<T> gv = buf[c * PLB + pidx(n)];
+        const cx<T> fv = buf[(TK + c) * PLB + pidx(n)];
+        const T k = a.kyp[j];
+        v = cx<T>{(gv.x - k * fv.y) * s, (gv.y + k * fv.x) * s};
+        if (cmp == 2 && j == 0 && k0 + c == 0) v = cx<T>{0, 0};   // rw mean removal (comp1.f90:172)
+      }
+      if (cmp < 3) {
+        a.R[cmp][(size_t)z * splane + (size_t)j * MX + k0 + c] = v;
+      } else {
+        spec[c * PLS + pidx(j)] = v;
+      }
+    }
+    if (cmp == 3) {
+      __syncthreads();
+      fft_waves<T, NY, true, TK, NT / 64>(spec, tws, tid);
+      __syncthreads();
+      cx<T>* o = a.Tre + (size_t)z * splane + k0;
+#pragma unroll
+      for (int i = tid; i < TK * NY; i += NT) {
+        const int c = i % TK, y = i / TK;
+        o[(size_t)y * MX + c] = spec[c * PLS + pidx(y)];
+      }
+    }
   }
 }
 
@@ -701,8 +754,12 @@ __device__ __forceinline__ void st2(cx<T>* p, cx<double> v) {
   *p = cx<T>{(T)v.x, (T)v.y};
 }
 
+// The z sweeps are latency bound (one dependent step per level), so each
+// thread loads the inputs of ZU levels before running their ZU recurrence
+// steps: ZU levels of loads are in flight per memory round trip.
 template <typename T>
 __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
+  constexpr int ZU = 4;
   const int modes = a.mx * a.ny;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= modes) return;
@@ -715,79 +772,132 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   const int nz = a.nz;
   const size_t pl = (size_t)modes;
   const bool sub = (a.istage != 1);
+  cx<T>* __restrict__ U0 = a.U[0];
+  cx<T>* __restrict__ U1 = a.U[1];
+  cx<T>* __restrict__ U2 = a.U[2];
+  cx<T>* __restrict__ R0 = a.Ro[0];
+  cx<T>* __restrict__ R1 = a.Ro[1];
+  cx<T>* __restrict__ R2 = a.Ro[2];
+  const cx<T>* __restrict__ N0 = a.Rn[0];
+  const cx<T>* __restrict__ N1 = a.Rn[1];
+  const cx<T>* __restrict__ N2 = a.Rn[2];
+  double* __restrict__ sd = a.sd;
+  cx<double>* __restrict__ sr = a.sr;
   // forward sweep (comprhs.f90:17-35, press.F90:87-133, tridv.f90:200-209)
   cx<double> wm_prev = {0, 0}, rw_prev = {0, 0}, r_prev = {0, 0};
   double d_prev = 0.0;
-  for (int z = 1; z <= nz; ++z) {
-    const size_t o = (size_t)z * pl + m;
-    cx<double> u = ld2(a.U[0] + o), v = ld2(a.U[1] + o), w = ld2(a.U[2] + o);
-    const cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
-    if (sub) {                                   // comp1.f90:126-133
-      u = csub(u, cscale(ld2(a.Ro[0] + o), a.dtab2));
-      v = csub(v, cscale(ld2(a.Ro[1] + o), a.dtab2));
-      if (z != nz) w = csub(w, cscale(ld2(a.Ro[2] + o), a.dtab2));
-      st2(a.U[0] + o, u);
-      st2(a.U[1] + o, v);
-      st2(a.U[2] + o, w);
+  for (int zb = 1; zb <= nz; zb += ZU) {
+    cx<T> lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU], lou[ZU], lov[ZU], low[ZU];
+#pragma unroll
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zb + q;
+      if (z <= nz) {
+        const size_t o = (size_t)z * pl + m;
+        lu[q] = U0[o]; lv[q] = U1[o]; lw[q] = U2[o];
+        lru[q] = N0[o]; lrv[q] = N1[o]; lrw[q] = N2[o];
+        if (sub) { lou[q] = R0[o]; lov[q] = R1[o]; low[q] = R2[o]; }
+      }
     }
-    const cx<double> fx = cadd(ru, cscale(u, a.dtabm));
-    const cx<double> fy = cadd(rv, cscale(v, a.dtabm));
-    cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
-    const cx<double> dw = cadd(cscale(csub(w, wm_prev), a.dtabm), csub(rw, rw_prev));
-    rhs = cadd(rhs, cscale(dw, a.dzm));
-    wm_prev = w;
-    rw_prev = rw;
-    double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
-    double lo = 1.0;
-    cx<double> r = cscale(rhs, dz2);
-    if (zero_mode) {                             // press.F90:125-132
-      diag = 1.0;
-      lo = 0.0;
-      r = cx<double>{0, 0};
+#pragma unroll
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zb + q;
+      if (z > nz) break;
+      const size_t o = (size_t)z * pl + m;
+      cx<double> u = {(double)lu[q].x, (double)lu[q].y};
+      cx<double> v = {(double)lv[q].x, (double)lv[q].y};
+      cx<double> w = {(double)lw[q].x, (double)lw[q].y};
+      if (sub) {                                 // comp1.f90:126-133
+        u = csub(u, cscale(cx<double>{(double)lou[q].x, (double)lou[q].y}, a.dtab2));
+        v = csub(v, cscale(cx<double>{(double)lov[q].x, (double)lov[q].y}, a.dtab2));
+        if (z != nz) w = csub(w, cscale(cx<double>{(double)low[q].x, (double)low[q].y}, a.dtab2));
+        st2(U0 + o, u);
+        st2(U1 + o, v);
+        st2(U2 + o, w);
+      }
+      const cx<double> ru = {(double)lru[q].x, (double)lru[q].y};
+      const cx<double> rv = {(double)lrv[q].x, (double)lrv[q].y};
+      const cx<double> rw = {(double)lrw[q].x, (double)lrw[q].y};
+      const cx<double> fx = cadd(ru, cscale(u, a.dtabm));
+      const cx<double> fy = cadd(rv, cscale(v, a.dtabm));
+      cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
+      const cx<double> dw = cadd(cscale(csub(w, wm_prev), a.dtabm), csub(rw, rw_prev));
+      rhs = cadd(rhs, cscale(dw, a.dzm));
+      wm_prev = w;
+      rw_prev = rw;
+      double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
+      double lo = 1.0;
+      cx<double> r = cscale(rhs, dz2);
+      if (zero_mode) {                           // press.F90:125-132
+        diag = 1.0;
+        lo = 0.0;
+        r = cx<double>{0, 0};
+      }
+      double d;
+      if (z == 1) {
+        d = 1.0 / diag;
+      } else {
+        const double fac = lo * d_prev;
+        d = 1.0 / (diag - fac * lo);             // a(i-1) = 1 (0 for the zero mode)
+        r = csub(r, cscale(r_prev, fac));
+      }
+      sd[o] = d;
+      sr[o] = r;
+      d_prev = d;
+      r_prev = r;
     }
-    double d;
-    if (z == 1) {
-      d = 1.0 / diag;
-    } else {
-      const double fac = lo * d_prev;
-      d = 1.0 / (diag - fac * lo);               // a(i-1) = 1 (0 for the zero mode)
-      r = csub(r, cscale(r_prev, fac));
-    }
-    a.sd[o] = d;
-    a.sr[o] = r;
-    d_prev = d;
-    r_prev = r;
   }
   // backward sweep (tridv.f90:210-219) fused with comp2 (comp2.f90:238-263)
   const double up_c = zero_mode ? 0.0 : 1.0;
   cx<double> p_next = {0, 0};
-  for (int z = nz; z >= 1; --z) {
-    const size_t o = (size_t)z * pl + m;
-    const double d = a.sd[o];
-    const cx<double> r = a.sr[o];
-    const cx<double> u = ld2(a.U[0] + o), v = ld2(a.U[1] + o), w = ld2(a.U[2] + o);
-    cx<double> ru = ld2(a.Rn[0] + o), rv = ld2(a.Rn[1] + o), rw = ld2(a.Rn[2] + o);
-    cx<double> p = (z == nz) ? cscale(r, d) : cscale(csub(r, cscale(p_next, up_c)), d);
-    if (zero_mode) p = cx<double>{0, 0};         // press.F90:151-152
-    ru = csub(ru, cmuli(cscale(p, kxp)));        // comp2.f90:241-242
-    rv = csub(rv, cmuli(cscale(p, kyp)));        // comp2.f90:246-247
-    cx<double> wn = {0, 0};
-    if (z != nz) {
-      rw = csub(rw, cscale(csub(p_next, p), a.dzm));   // comp2.f90:252
-      if (zero_mode) rw = cx<double>{0, 0};            // comp2.f90:253
-      wn = cadd(w, cscale(rw, a.dtab1));               // comp2.f90:260
-      if (zero_mode) wn = cx<double>{0, 0};            // comp2.f90:262
-    } else {
-      rw = cx<double>{0, 0};
+  for (int zt = nz; zt >= 1; zt -= ZU) {
+    double ld[ZU];
+    cx<double> lr[ZU];
+    cx<T> lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU];
+#pragma unroll
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zt - q;
+      if (z >= 1) {
+        const size_t o = (size_t)z * pl + m;
+        ld[q] = sd[o]; lr[q] = sr[o];
+        lu[q] = U0[o]; lv[q] = U1[o]; lw[q] = U2[o];
+        lru[q] = N0[o]; lrv[q] = N1[o]; lrw[q] = N2[o];
+      }
     }
-    st2(a.U[0] + o, cadd(u, cscale(ru, a.dtab1)));    // comp2.f90:257-258
-    st2(a.U[1] + o, cadd(v, cscale(rv, a.dtab1)));
-    st2(a.U[2] + o, wn);
-    st2(a.Ro[0] + o, ru);
-    st2(a.Ro[1] + o, rv);
-    st2(a.Ro[2] + o, rw);
-    a.P[o] = p;
-    p_next = p;
+#pragma unroll
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zt - q;
+      if (z < 1) break;
+      const size_t o = (size_t)z * pl + m;
+      const double d = ld[q];
+      const cx<double> r = lr[q];
+      const cx<double> u = {(double)lu[q].x, (double)lu[q].y};
+      const cx<double> v = {(double)lv[q].x, (double)lv[q].y};
+      const cx<double> w = {(double)lw[q].x, (double)lw[q].y};
+      cx<double> ru = {(double)lru[q].x, (double)lru[q].y};
+      cx<double> rv = {(double)lrv[q].x, (double)lrv[q].y};
+      cx<double> rw = {(double)lrw[q].x, (double)lrw[q].y};
+      cx<double> p = (z == nz) ? cscale(r, d) : cscale(csub(r, cscale(p_next, up_c)), d);
+      if (zero_mode) p = cx<double>{0, 0};       // press.F90:151-152
+      ru = csub(ru, cmuli(cscale(p, kxp)));      // comp2.f90:241-242
+      rv = csub(rv, cmuli(cscale(p, kyp)));      // comp2.f90:246-247
+      cx<double> wn = {0, 0};
+      if (z != nz) {
+        rw = csub(rw, cscale(csub(p_next, p), a.dzm));   // comp2.f90:252
+        if (zero_mode) rw = cx<double>{0, 0};            // comp2.f90:253
+        wn = cadd(w, cscale(rw, a.dtab1));               // comp2.f90:260
+        if (zero_mode) wn = cx<double>{0, 0};            // comp2.f90:262
+      } else {
+        rw = cx<double>{0, 0};
+      }
+      st2(U0 + o, cadd(u, cscale(ru, a.dtab1)));        // comp2.f90:257-258
+      st2(U1 + o, cadd(v, cscale(rv, a.dtab1)));
+      st2(U2 + o, wn);
+      st2(R0 + o, ru);
+      st2(R1 + o, rv);
+      st2(R2 + o, rw);
+      a.P[o] = p;
+      p_next = p;
+    }
   }
 }
 
@@ -813,7 +923,7 @@ struct ERowArgs {
 template <typename T, int NX>
 __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
   constexpr int MX = NX / 2, NT = MX, NY = NX;
-  __shared__ cx<T> buf[NX];
+  __shared__ cx<T> buf[padlen(NX)];
   __shared__ cx<T> tw[NX];
   const int tid = threadIdx.x;
   const int y0 = blockIdx.x, y1 = blockIdx.x + NY / 2;
@@ -840,10 +950,10 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
     const cx<T> av = s0[k], bv = s1[k];
     if (k == 0) {
       buf[0] = cx<T>{av.x, bv.x};
-      buf[MX] = cx<T>{0, 0};
+      buf[pidx(MX)] = cx<T>{0, 0};
     } else {
-      buf[k] = cx<T>{av.x - bv.y, av.y + bv.x};
-      buf[NX - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+      buf[pidx(k)] = cx<T>{av.x - bv.y, av.y + bv.x};
+      buf[pidx(NX - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
     }
     __syncthreads();
     fft_run<T, NX, true, 1, NT>(buf, tw, tid);
@@ -851,7 +961,7 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int x = tid + p * NT;
-      const cx<T> r = buf[x];
+      const cx<T> r = buf[pidx(x)];
       T* pe0 = e + (size_t)y0 * NX + x;
       T* pe1 = e + (size_t)y1 * NX + x;
       T* pr0 = re + (size_t)y0 * NX + x;
@@ -885,13 +995,13 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
   }
   // x-forward of the two rows
 #pragma unroll
-  for (int p = 0; p < 2; ++p) buf[tid + p * NT] = cx<T>{ea[p], eb[p]};
+  for (int p = 0; p < 2; ++p) buf[pidx(tid + p * NT)] = cx<T>{ea[p], eb[p]};
   __syncthreads();
   fft_run<T, NX, false, 1, NT>(buf, tw, tid);
   const T s = T(0.5) / T(NX);
   const int k = tid;
-  const cx<T> zk = buf[k];
-  const cx<T> zn = buf[k == 0 ? 0 : NX - k];
+  const cx<T> zk = buf[pidx(k)];
+  const cx<T> zn = buf[pidx(k == 0 ? 0 : NX - k)];
   cx<T>* o0 = a.Ex + (size_t)z * splane + (size_t)y0 * MX;
   cx<T>* o1 = a.Ex + (size_t)z * splane + (size_t)y1 * MX;
   o0[k] = cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s};
@@ -906,7 +1016,7 @@ template <typename T, int NX>
 __global__ void __launch_bounds__(NX / 2) k_rows_fwd(const double* __restrict__ in, cx<T>* out,
                                                       const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = MX, NY = NX;
-  __shared__ cx<T> buf[NX];
+  __shared__ cx<T> buf[padlen(NX)];
   __shared__ cx<T> tw[NX];
   const int tid = threadIdx.x;
   load_twiddles(tw, twg, NX, tid, NT);
@@ -916,13 +1026,13 @@ __global__ void __launch_bounds__(NX / 2) k_rows_fwd(const double* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int x = tid + q * NT;
-    buf[x] = cx<T>{(T)p[(size_t)y0 * NX + x], (T)p[(size_t)y1 * NX + x]};
+    buf[pidx(x)] = cx<T>{(T)p[(size_t)y0 * NX + x], (T)p[(size_t)y1 * NX + x]};
   }
   __syncthreads();
   fft_run<T, NX, false, 1, NT>(buf, tw, tid);
   const T s = T(0.5) / T(NX);
   const int k = tid;
-  const cx<T> zk = buf[k], zn = buf[k == 0 ? 0 : NX - k];
+  const cx<T> zk = buf[pidx(k)], zn = buf[pidx(k == 0 ? 0 : NX - k)];
   cx<T>* o = out + (size_t)L * NY * MX;
   o[(size_t)y0 * MX + k] = cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s};
   o[(size_t)y1 * MX + k] = cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s};
@@ -933,7 +1043,7 @@ template <typename T, int NX>
 __global__ void __launch_bounds__(NX / 2) k_rows_inv(const cx<T>* __restrict__ in, double* out,
                                                       const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = MX, NY = NX;
-  __shared__ cx<T> buf[NX];
+  __shared__ cx<T> buf[padlen(NX)];
   __shared__ cx<T> tw[NX];
   const int tid = threadIdx.x;
   load_twiddles(tw, twg, NX, tid, NT);
@@ -945,10 +1055,10 @@ __global__ void __launch_bounds__(NX / 2) k_rows_inv(const cx<T>* __restrict__ i
   const cx<T> av = s0[k], bv = s1[k];
   if (k == 0) {
     buf[0] = cx<T>{av.x, bv.x};
-    buf[MX] = cx<T>{0, 0};
+    buf[pidx(MX)] = cx<T>{0, 0};
   } else {
-    buf[k] = cx<T>{av.x - bv.y, av.y + bv.x};
-    buf[NX - k] = cx<T>{av.x + bv.y, bv.x - av.y};
+    buf[pidx(k)] = cx<T>{av.x - bv.y, av.y + bv.x};
+    buf[pidx(NX - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
   }
   __syncthreads();
   fft_run<T, NX, true, 1, NT>(buf, tw, tid);
@@ -956,8 +1066,8 @@ __global__ void __launch_bounds__(NX / 2) k_rows_inv(const cx<T>* __restrict__ i
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int x = tid + q * NT;
-    p[(size_t)y0 * NX + x] = (double)buf[x].x;
-    p[(size_t)y1 * NX + x] = (double)buf[x].y;
+    p[(size_t)y0 * NX + x] = (double)buf[pidx(x)].x;
+    p[(size_t)y1 * NX + x] = (double)buf[pidx(x)].y;
   }
 }
 
@@ -967,27 +1077,29 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
                                                      const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = 256;
   constexpr int TK = ColShape<T, NX, NY>::TK;
-  __shared__ cx<T> buf[TK * NY];
+  __shared__ cx<T> buf[TK * padlen(NY)];
   __shared__ cx<T> tw[NY];
   const int tid = threadIdx.x;
   load_twiddles(tw, twg, NY, tid, NT);
   const int k0 = blockIdx.x * TK;
   const int L = l0 + blockIdx.y;
   const cx<T>* src = in + (size_t)L * NY * MX + k0;
+#pragma unroll
   for (int i = tid; i < TK * NY; i += NT) {
     const int c = i % TK, j = i / TK;
     cx<T> v = src[(size_t)j * MX + c];
     if (INV && j == NY / 2) v = cx<T>{0, 0};
-    buf[c * NY + j] = v;
+    buf[c * padlen(NY) + pidx(j)] = v;
   }
   __syncthreads();
   fft_waves<T, NY, INV, TK, NT / 64>(buf, tw, tid);
   __syncthreads();
   cx<T>* dst = out + (size_t)L * NY * MX + k0;
   const T s = INV ? T(1) : T(1) / T(NY);
+#pragma unroll
   for (int i = tid; i < TK * NY; i += NT) {
     const int c = i % TK, j = i / TK;
-    cx<T> v = cscale(buf[c * NY + j], s);
+    cx<T> v = cscale(buf[c * padlen(NY) + pidx(j)], s);
     if (!INV && j == NY / 2) v = cx<T>{0, 0};
     dst[(size_t)j * MX + c] = v;
   }

@@ -49,7 +49,7 @@ struct Engine;
 
 struct cf_ctx {
   cf_params p{};
-  int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16;
+  int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16, ncu = 256;
   bool fp32 = false;
   hipStream_t stream = nullptr;
   std::string err;
@@ -276,8 +276,9 @@ struct EngineT : Engine {
       a.twNY = cptr(ctx->twNX);
       a.twNYB = cptr(ctx->twNXB);
       a.nz = nz;
-      hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(MX / CS::TK, nz, 4), dim3(CS::NT), 0, s,
-                         a);
+      const int ntiles = (MX / CS::TK) * nz * 4;
+      const int grid = std::min(ntiles, 2 * ctx->ncu);
+      hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(grid), dim3(CS::NT), 0, s, a);
     }
     mark();
     {
@@ -510,6 +511,9 @@ int cf_create(const cf_params* p, cf_ctx** out) {
     delete ctx;
     return CF_E_HIP;
   }
+  if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+      ctx->ncu <= 0)
+    ctx->ncu = 256;
   ctx->eng = ctx->fp32 ? make_engine_t<float>(nx) : make_engine_t<double>(nx);
   int rc = ctx->eng->alloc(ctx);
   if (rc != CF_OK) {
