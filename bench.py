@@ -9,12 +9,16 @@ and update); the FP64 plain path is timed beside it.  Fields are resident in
 HBM before the timed region; the timed region is K full steps on the library
 stream bracketed by stream synchronisation, measured with HIP events.
 
-One JSON line is printed.  ``roofline`` prices the step at the algorithmic
-HBM bytes per grid-point step (FP32 path: 85 FP32 + 12 FP64 array passes =
-436 B; FP64 path: 97 x 8 = 776 B) against the 8 TB/s HBM3E peak, and names the
-dominant kernel with its measured per-launch time.  ``cpu_baseline`` times
-the FP64 numpy oracle (the CPU restatement of the reference stage) on one
-stage of the same 256x256x128 workload, on one host core.
+One JSON line is printed.  ``roofline`` prices the dominant kernel (the
+row-tendency kernel k_rows): the bytes its launch must move -- the 8 y-synthesised
+x-spectral input rows and the 8 flux output rows on the 3/2 grid, each read or
+written once, 24 * N * sizeof(real) per launch for N = nnx*nny*nnz -- divided by
+its mean launch time from HIP events on the library stream, against the 8 TB/s
+HBM3E peak.  ``step_roofline`` prices the whole step at the compulsory bytes of
+the persistent arrays (FP32 path: 85 FP32 + 12 FP64 array passes = 436 B per
+grid-point step; FP64 path: 97 x 8 = 776 B).  ``cpu_baseline`` times the FP64
+numpy oracle (the CPU restatement of the reference stage) on one stage of the
+same 256x256x128 workload, on one host core.
 """
 from __future__ import annotations
 
@@ -30,6 +34,34 @@ sys.path.insert(0, os.path.join(ROOT, "paes-channelflow_amd"))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_GPSTEP = {"fp32": 436.0, "fp64": 776.0}
+
+
+def kernel_bytes(name, n, s, istage=2):
+    """Bytes one launch of kernel ``name`` must read and write, for N = n grid
+    points and s-byte reals (complex = 2s; band spectra and x-spectral rows of
+    the nnx x nny plane hold N*s bytes per field, 3/2-grid rows 1.5*N*s)."""
+    return {
+        # U, V, W band spectra + x-spectral e in; u,uy,v,vy,w,wy,e,ey rows out
+        "k_col_synth": 4 * n * s + 12 * n * s,
+        # 8 synthesis rows in, 8 flux rows (Gu,Fyu,Gv,Fyv,Gw,Fyw,Ge,Fye) out
+        "k_rows": 12 * n * s + 12 * n * s,
+        # 8 flux rows in; ru, rv, rw band spectra + x-spectral re out
+        "k_col_anal": 12 * n * s + 4 * n * s,
+        # U,V,W, R_old, R_new in; U,V,W, R out; FP64 pressure spectrum out
+        "k_press": 15 * n * s + 8 * n,
+        # x-spectral re, e, re in; e, re, x-spectral e out
+        "k_erows": (6 if istage > 1 else 5) * n * s,
+    }.get(name)
+
+# HBM bytes per dispatch from the FETCH_SIZE and WRITE_SIZE counters (KiB per
+# dispatch x 1024) for the 256x256x128 FP32 workload.  FETCH_SIZE counts
+# contiguous full-line reads at half their size (a 64 MiB contiguous read by
+# k_cast reads back as 32 MiB), so k_rows' row reads take a factor 2;
+# WRITE_SIZE is 1:1 (a 256 MiB fill reads back as 256 MiB).
+# profiles/k_rows_pmc_256x256x128.md holds the runs.
+MEASURED_TRAFFIC = {
+    ("k_rows", 256, 128, "fp32"): round((2 * 198014.8 + 393262.8) * 1024),
+}
 
 
 def parse():
@@ -52,7 +84,6 @@ def parse():
 def run_gpu(args, precision_name):
     import numpy as np
     import channelflow as C
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     nx, nz = args.nx, args.nz
     z0 = C.z0_of_retau(args.re_tau)
     dz = 2.0 / nz
@@ -138,7 +169,12 @@ def main():
     kern = r32["per_kernel"]
     dom = max((k for k in kern if not k.startswith("(")), key=lambda k: kern[k])
     step_ms = r32["ms"] / args.steps
-    achieved = BYTES_PER_GPSTEP["fp32"] * value / 1e9
+    step_achieved = BYTES_PER_GPSTEP["fp32"] * value / 1e9
+    dom_bytes = kernel_bytes(dom, pts, 4)
+    achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
+    per_kernel_gbs = {k: kernel_bytes(k, pts, 4) / (t * 1e-3) / 1e9
+                      for k, t in kern.items() if kernel_bytes(k, pts, 4) and t > 0}
+    traffic = MEASURED_TRAFFIC.get((dom, args.nx, args.nz, "fp32"))
     line = {
         "metric": "grid-pt RK3 steps/s @256^2x128: FP32(+FP64 pressure) TKE-SGS path; % HBM peak",
         "value": value,
@@ -166,13 +202,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "basis": "436 algorithmic HBM bytes per grid-point RK3 step x grid-pt steps/s, "
-                     "time from HIP events over the step loop",
-            "dominant_kernel": dom,
-            "dominant_kernel_ms_per_launch": kern[dom],
-            "kernel_ms_per_stage": kern,
+            "traffic": traffic,
+            "kernel": dom,
+            "bytes_per_launch": dom_bytes,
+            "ms_per_launch": kern[dom],
+            "basis": f"{dom}: 24 x N x 4 B per launch (3/2-grid x-spectral rows in and out, "
+                     "once each) / mean launch time from HIP events on the library stream",
         },
+        "step_roofline": {
+            "achieved": step_achieved,
+            "frac": step_achieved / HBM_PEAK_GBS,
+            "basis": "436 compulsory HBM bytes per grid-point RK3 step x grid-pt steps/s",
+        },
+        "kernel_ms_per_stage": kern,
+        "kernel_gbs": per_kernel_gbs,
         "fp64_plain": None,
         "finite": r32["finite"],
     }
