@@ -78,6 +78,7 @@ class Params:
     alk: float | None = None  # SGS length scale (allocate.f90:61, never set; None -> dsl)
     ugal: float = 0.0         # storage module globals, zero in the reference run
     vgal: float = 0.0
+    closure: object = None    # learned-closure hook (sgs_mlp.LearnedClosure); None = TKE
 
     def __post_init__(self):
         nnx, nny, nnz = self.nnx, self.nny, self.nnz
@@ -438,6 +439,8 @@ def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
     n = pr.nnz
     xl, yl = pr.xl, pr.yl
     F = sgs_fields(st, pr)
+    if pr.closure is not None:
+        pr.closure.apply(F, pr, st)          # learned stresses (no reference counterpart)
     ub, vb, wb, eb = F["ub"], F["vb"], F["wb"], F["eb"]
     ux, uy, vx, vy, wz = F["ux"], F["uy"], F["vx"], F["vy"], F["wz"]
     ex, ey, xkk = F["ex"], F["ey"], F["xkk"]
