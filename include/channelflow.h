@@ -143,6 +143,84 @@ int64_t cf_device_bytes(const cf_ctx* ctx);
 /* Text of the last error of this context (or of cf_create when ctx is NULL). */
 const char* cf_last_error(const cf_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Learned SGS closure (no counterpart in the reference).
+ *
+ * A pointwise MLP 11 -> 64 -> 64 -> 64 -> 6 (GELU, tanh form) maps the
+ * resolved velocity gradients at a cell centre (ux uy uz vx vy vz wx wy wz),
+ * the filter width dsl (les.F90:118) and the wall distance min(z, zl-z) to
+ * the SGS stresses (t11 t22 t33 t12 t13 t23).  It replaces the stresses of
+ * compsgs1 (compsgs1.f90:104-156) at the same staggered locations, with the
+ * TKE closure as fallback wherever the prediction is not realizable or
+ * backscatters.  BF16 weights and activations on MFMA with FP32
+ * accumulation; FP32 master weights and Adam (beta 0.9/0.999, eps 1e-8).
+ *
+ * Inputs are standardised x = (f - in_mu)/in_sd and outputs de-standardised
+ * tau = y*out_sd + out_mu.  Parameters are one flat float array in the
+ * order W1[64][11], b1[64], W2[64][64], b2, W3[64][64], b3, W4[6][64],
+ * b4[6] (CF_MLP_NPARAM floats).
+ * ------------------------------------------------------------------------ */
+typedef struct cf_mlp cf_mlp;
+#define CF_MLP_NIN 11
+#define CF_MLP_NOUT 6
+#define CF_MLP_NPARAM 9478
+
+/* Network object on a device: parameters, optimiser state, training samples. */
+int cf_mlp_create(int32_t device, cf_mlp** out);
+int cf_mlp_destroy(cf_mlp* m);
+
+/* Flat parameters (CF_MLP_NPARAM floats) in and out. */
+int cf_mlp_set_params(cf_mlp* m, const float* params);
+int cf_mlp_get_params(cf_mlp* m, float* params);
+
+/* Standardisation: in_mu, in_sd (11 each), out_mu, out_sd (6 each); sd > 0.
+ * Must be set before cf_mlp_put_samples, which stores samples standardised. */
+int cf_mlp_set_norm(cf_mlp* m, const float* in_mu, const float* in_sd, const float* out_mu,
+                    const float* out_sd);
+int cf_mlp_get_norm(cf_mlp* m, float* in_mu, float* in_sd, float* out_mu, float* out_sd);
+
+/* Network file: "CFMLP001", int32 {11, 64, 6, 9478}, float in_mu[11],
+ * in_sd[11], out_mu[6], out_sd[6], params[9478], little endian. */
+int cf_mlp_save(cf_mlp* m, const char* path);
+int cf_mlp_load(cf_mlp* m, const char* path);
+
+/* Inference on the device: feats [n][11] raw -> stresses [n][6] raw. */
+int cf_mlp_forward(cf_mlp* m, int64_t n, const float* feats, float* stresses);
+
+/* Training samples on the device: room for capacity samples (discards any
+ * stored), then raw features [n][11] and target stresses [n][6] at offset. */
+int cf_mlp_reserve(cf_mlp* m, int64_t capacity);
+int cf_mlp_put_samples(cf_mlp* m, int64_t offset, int64_t n, const float* feats,
+                       const float* targets);
+
+/* Mean squared error over the standardised outputs of the stored samples
+ * offset..offset+n-1 and its gradient (CF_MLP_NPARAM floats); no update. */
+int cf_mlp_loss_grad(cf_mlp* m, int64_t offset, int64_t n, float* loss, float* grad);
+
+/* nsteps Adam steps at learning rate lr.  batch > 0: each step draws batch
+ * samples with replacement from the stored ones (counter-based, from seed
+ * and the step count); batch == 0: full batch in order.  losses (nsteps
+ * floats, may be NULL) receives each step's loss before its update. */
+int cf_mlp_train(cf_mlp* m, int32_t nsteps, int64_t batch, float lr, uint64_t seed, float* losses);
+int cf_mlp_reset_optimizer(cf_mlp* m);
+const char* cf_mlp_last_error(const cf_mlp* m);
+
+/* Use network m as the SGS closure of ctx from the next stage on (its
+ * current parameters are copied); NULL restores the TKE closure.  Needs
+ * nnx in {128, 256, 512}. */
+int cf_set_closure(cf_ctx* ctx, const cf_mlp* m);
+
+/* Cells where the network's stress was used, and cells evaluated, since the
+ * previous call (then reset): counts[2]. */
+int cf_get_closure_counts(cf_ctx* ctx, int64_t* counts);
+
+/* Training data: box filter of the current state by fx x fy x fz fine cells
+ * (fx, fy in {1,2,4,8}, fz in {1,2,4}, dividing the grid) into
+ * out[11][ncz][ncy][ncx] doubles: u, v, w (cell centre), w (coarse face
+ * K+1, trapezoid over fz+1 fine faces, 0 at the top wall), e (cell centre),
+ * uu, vv, ww, uv, uw, vw box means of the cell-centred fine velocities. */
+int cf_filter_snapshot(cf_ctx* ctx, int32_t fx, int32_t fy, int32_t fz, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,7 +39,15 @@ EXPORTS = (
     "cf_get_comp1_tendency", "cf_stage", "cf_step", "cf_set_dt", "cf_sync", "cf_get_divz",
     "cf_get_scalars", "cf_compmn", "cf_time_steps", "cf_kernel_name", "cf_device_bytes",
     "cf_last_error",
+    # learned SGS closure
+    "cf_mlp_create", "cf_mlp_destroy", "cf_mlp_set_params", "cf_mlp_get_params", "cf_mlp_set_norm",
+    "cf_mlp_get_norm", "cf_mlp_save", "cf_mlp_load", "cf_mlp_forward", "cf_mlp_reserve",
+    "cf_mlp_put_samples", "cf_mlp_loss_grad", "cf_mlp_train", "cf_mlp_reset_optimizer",
+    "cf_mlp_last_error", "cf_set_closure", "cf_get_closure_counts", "cf_filter_snapshot",
 )
+CF_MLP_NIN = 11
+CF_MLP_NOUT = 6
+CF_MLP_NPARAM = 9478
 
 
 class CfParams(ctypes.Structure):
@@ -95,8 +103,29 @@ def load_library(path: str = LIB_PATH):
     lib.cf_device_bytes.restype = ctypes.c_int64
     lib.cf_last_error.argtypes = [vp]
     lib.cf_last_error.restype = ctypes.c_char_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    i64 = ctypes.c_int64
+    lib.cf_mlp_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
+    lib.cf_mlp_destroy.argtypes = [vp]
+    lib.cf_mlp_set_params.argtypes = [vp, fp]
+    lib.cf_mlp_get_params.argtypes = [vp, fp]
+    lib.cf_mlp_set_norm.argtypes = [vp, fp, fp, fp, fp]
+    lib.cf_mlp_get_norm.argtypes = [vp, fp, fp, fp, fp]
+    lib.cf_mlp_save.argtypes = [vp, ctypes.c_char_p]
+    lib.cf_mlp_load.argtypes = [vp, ctypes.c_char_p]
+    lib.cf_mlp_forward.argtypes = [vp, i64, fp, fp]
+    lib.cf_mlp_reserve.argtypes = [vp, i64]
+    lib.cf_mlp_put_samples.argtypes = [vp, i64, i64, fp, fp]
+    lib.cf_mlp_loss_grad.argtypes = [vp, i64, i64, fp, fp]
+    lib.cf_mlp_train.argtypes = [vp, ctypes.c_int32, i64, ctypes.c_float, ctypes.c_uint64, fp]
+    lib.cf_mlp_reset_optimizer.argtypes = [vp]
+    lib.cf_mlp_last_error.argtypes = [vp]
+    lib.cf_mlp_last_error.restype = ctypes.c_char_p
+    lib.cf_set_closure.argtypes = [vp, vp]
+    lib.cf_get_closure_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.cf_filter_snapshot.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp]
     for name in EXPORTS:
-        if name not in ("cf_kernel_name", "cf_device_bytes", "cf_last_error"):
+        if name not in ("cf_kernel_name", "cf_device_bytes", "cf_last_error", "cf_mlp_last_error"):
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -239,3 +268,119 @@ class ChannelFlow:
 
     def device_bytes(self) -> int:
         return int(self.lib.cf_device_bytes(self._h))
+
+    def set_closure(self, net: "SgsNet | None"):
+        """Use the learned closure ``net`` from the next stage on (None: TKE)."""
+        self._check(self.lib.cf_set_closure(self._h, net._h if net is not None else None))
+
+    def closure_counts(self):
+        """(cells that used the network's stress, cells evaluated) since the last call."""
+        out = (ctypes.c_int64 * 2)()
+        self._check(self.lib.cf_get_closure_counts(self._h, out))
+        return int(out[0]), int(out[1])
+
+    def filter_snapshot(self, fx: int, fy: int, fz: int):
+        """Box filter of the current state onto the (nnx/fx, nny/fy, nnz/fz)
+        grid: dict of u, v, wc, wf, e, uu, vv, ww, uv, uw, vw arrays (ncz, ncy, ncx)."""
+        nz, ny, nx = self.shape
+        shp = (nz // fz, ny // fy, nx // fx)
+        out = np.empty((11,) + shp)
+        self._check(self.lib.cf_filter_snapshot(self._h, fx, fy, fz, _ptr(out)))
+        return dict(zip(FILTER_NAMES, out))
+
+
+FILTER_NAMES = ("u", "v", "wc", "wf", "e", "uu", "vv", "ww", "uv", "uw", "vw")
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class SgsNet:
+    """The learned SGS closure network on a device (a cf_mlp): parameters,
+    standardisation, Adam state and training samples."""
+
+    def __init__(self, device: int = 0, path: str | None = None):
+        self.lib = load_library()
+        self._h = ctypes.c_void_p()
+        rc = self.lib.cf_mlp_create(device, ctypes.byref(self._h))
+        if rc != CF_OK:
+            raise ChannelFlowError(rc, self.lib.cf_mlp_last_error(None).decode())
+        if path is not None:
+            self.load(path)
+
+    def _check(self, rc):
+        if rc != CF_OK:
+            raise ChannelFlowError(rc, self.lib.cf_mlp_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self.lib.cf_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f32(a, shape=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if shape is not None and a.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {a.shape}")
+        return a
+
+    def set_params(self, flat):
+        p = self._f32(flat, (CF_MLP_NPARAM,))
+        self._check(self.lib.cf_mlp_set_params(self._h, _fptr(p)))
+
+    def get_params(self):
+        p = np.empty(CF_MLP_NPARAM, dtype=np.float32)
+        self._check(self.lib.cf_mlp_get_params(self._h, _fptr(p)))
+        return p
+
+    def set_norm(self, in_mu, in_sd, out_mu, out_sd):
+        a = [self._f32(in_mu, (11,)), self._f32(in_sd, (11,)), self._f32(out_mu, (6,)), self._f32(out_sd, (6,))]
+        self._check(self.lib.cf_mlp_set_norm(self._h, *(_fptr(x) for x in a)))
+
+    def get_norm(self):
+        a = [np.empty(11, np.float32), np.empty(11, np.float32), np.empty(6, np.float32), np.empty(6, np.float32)]
+        self._check(self.lib.cf_mlp_get_norm(self._h, *(_fptr(x) for x in a)))
+        return tuple(a)
+
+    def save(self, path: str):
+        self._check(self.lib.cf_mlp_save(self._h, os.fsencode(path)))
+
+    def load(self, path: str):
+        self._check(self.lib.cf_mlp_load(self._h, os.fsencode(path)))
+
+    def forward(self, feats):
+        f = self._f32(feats)
+        n = f.shape[0]
+        out = np.empty((n, CF_MLP_NOUT), dtype=np.float32)
+        self._check(self.lib.cf_mlp_forward(self._h, n, _fptr(f), _fptr(out)))
+        return out
+
+    def reserve(self, capacity: int):
+        self._check(self.lib.cf_mlp_reserve(self._h, int(capacity)))
+
+    def put_samples(self, offset: int, feats, targets):
+        f = self._f32(feats)
+        t = self._f32(targets)
+        assert f.shape[0] == t.shape[0] and f.shape[1] == CF_MLP_NIN and t.shape[1] == CF_MLP_NOUT
+        self._check(self.lib.cf_mlp_put_samples(self._h, int(offset), f.shape[0], _fptr(f), _fptr(t)))
+
+    def loss_grad(self, offset: int, n: int):
+        loss = np.zeros(1, np.float32)
+        g = np.empty(CF_MLP_NPARAM, np.float32)
+        self._check(self.lib.cf_mlp_loss_grad(self._h, int(offset), int(n), _fptr(loss), _fptr(g)))
+        return float(loss[0]), g
+
+    def train(self, nsteps: int, batch: int, lr: float, seed: int = 0):
+        losses = np.empty(nsteps, np.float32)
+        self._check(self.lib.cf_mlp_train(self._h, nsteps, int(batch), lr, seed, _fptr(losses)))
+        return losses
+
+    def reset_optimizer(self):
+        self._check(self.lib.cf_mlp_reset_optimizer(self._h))

@@ -29,6 +29,7 @@
 //                its x-forward for the next stage
 #pragma once
 #include "cf_fft.hpp"
+#include "cf_mlp.hpp"
 
 namespace cf {
 
@@ -70,6 +71,7 @@ struct Consts {
   double ugal, vgal;
   double dtab1, dtab2, dtabm;
   double xymb, zody, vk, zl;
+  double dsl;                 // filter width (dx dy dz)^(1/3), les.F90:118
   int nz, istage, mnout, zchunk;
 };
 
@@ -606,6 +608,291 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     wxm = wx0; wym = wy0;
   }
   if (clampflag) atomicOr(a.flags, FLAG_ECLAMP);
+}
+
+// ---------------------------------------------------------------------------
+// k_rows_mlp: k_rows with the learned closure in place of the compsgs1
+// stresses (no reference counterpart; insertion point compsgs1.f90:104-156).
+//
+// The network runs at every 3/2-grid cell centre.  Its inputs:
+//   - ux uy vx vy at the level;
+//   - uz, vz = (u(k+1)-u(k-1))/(2dz);
+//   - wx, wy = the mean of the two faces;
+//   - wz = (w(k)-w(k-1))/dz;
+//   - dsl and the wall distance.
+// A cell uses the deviatoric network stress when the prediction is realizable
+// and dissipative, and the TKE stress otherwise.  A face takes the mean of its
+// two cells' tau13/tau23 when both cells passed, and the TKE face stress
+// otherwise.  The wall faces keep the wall model.
+//
+// The face iz and the w-flux at face iz need cell iz+1, so the window runs one
+// level ahead of k_rows: iteration iz synthesises u, v at iz+2, the horizontal
+// gradients and wx, wy at iz+1, and evaluates the network at cell iz+1.  Each
+// chunk starts with one warm-up iteration (level z0-1) that emits nothing.
+// Whole wavefronts are MFMA operands, so NXB must be a multiple of 64.
+// ---------------------------------------------------------------------------
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp::Net* __restrict__ gnet,
+                                                         unsigned long long* counts) {
+  using F = RowFFT<T, NX>;
+  static_assert(F::WAVE, "the learned closure needs 3/2-grid rows of whole wavefronts");
+  constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
+  constexpr int NYB = 3 * NX / 2;
+  __shared__ cx<T> buf[6 * padlen(NXB)];
+  __shared__ cx<T> tw[NXB];
+  __shared__ double red[RedSize<NT, 7>::N];
+  __shared__ mlp::Net net;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int y = blockIdx.x;
+  const Consts& c = a.c;
+  const int nz = c.nz;
+  const int z0 = 1 + blockIdx.y * c.zchunk;
+  int z1 = z0 + c.zchunk;
+  if (z1 > nz + 1) z1 = nz + 1;
+  const size_t plane = (size_t)NYB * MX;
+  const size_t rowoff = (size_t)y * MX;
+  auto row = [&](int fld, int L) -> const cx<T>* {
+    return a.B + (size_t)fld * a.bstride + (size_t)L * plane + rowoff;
+  };
+  auto clampF = [&](int F) { return F <= 0 ? 1 : (F >= nz ? nz - 1 : F); };
+  auto lvl = [&](int fld, int L) -> const cx<T>* { return (L >= 1 && L <= nz) ? row(fld, L) : nullptr; };
+  auto face = [&](int fld, int L) -> const cx<T>* { return (L >= 1 && L <= nz - 1) ? row(fld, L) : nullptr; };
+  enum { FU = 0, FUY, FV, FVY, FW, FWY, FE, FEY };
+  load_twiddles(tw, a.twNXB, NXB, tid, NT);
+  for (int q = tid; q < (int)(sizeof(mlp::Net) / 16); q += NT) ((uint4*)&net)[q] = ((const uint4*)gnet)[q];
+
+  const T emin = (T)c.emin;
+  const T alk = (T)c.alk, xkmax = (T)c.xkmax;
+  const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
+  const T ugal = (T)c.ugal, vgal = (T)c.vgal;
+  const T c_lo = (T)a.ws->c_lo, c_hi = (T)a.ws->c_hi;
+  const float dslf = (float)c.dsl, dzf = (float)c.dz, zlf = (float)c.zl;
+  const bool mn = c.mnout != 0;
+  int clampflag = 0;
+  unsigned long long nacc = 0;
+
+  auto xkk_of = [&](T e) {
+    T v = T(0.1) * alk * sqrt(e);
+    return v < xkmax ? v : xkmax;   // complen.f90:89
+  };
+  // the network at a cell with gradients g -> stress t, accepted or not
+  auto closure = [&](int k, const float (&g)[9], float (&t)[6]) -> bool {
+    const float zc = ((float)k - 0.5f) * dzf;
+    const float f[mlp::NIN] = {g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], dslf,
+                               fminf(zc, zlf - zc)};
+    mlp::infer(&net, f, t, lane);
+    return mlp::accept(t, g) && k >= 1 && k <= nz;
+  };
+
+  // window: u, v at iz-1..iz+2; w, e at faces iz-1..iz+1; horizontal
+  // gradients at iz, iz+1; wx, wy at faces iz-1..iz+1
+  T um, u0, up, upp, vm, v0, vp, vpp, wm, w0, wp, em, e0, ep;
+  T ux0, uy0, vx0, vy0, ux1, uy1, vx1, vy1;
+  T wxm, wym, wx0, wy0, wxp, wyp;
+  float t0[6];
+  bool ok0;
+  T fuw = 0, fvw = 0;                   // final tau13, tau23 at face iz-1
+  const int zs = z0 - 1;
+  {
+    const cx<T>* sa[6];
+    const cx<T>* sb[6];
+    const bool da[6] = {false, false, false, false, false, true};
+    const bool db[6] = {false, false, false, false, false, false};
+    T ra[6], rb[6];
+    sa[0] = lvl(FU, zs - 1); sb[0] = lvl(FV, zs - 1);
+    sa[1] = lvl(FU, zs);     sb[1] = lvl(FV, zs);
+    sa[2] = lvl(FU, zs + 1); sb[2] = lvl(FV, zs + 1);
+    sa[3] = face(FW, zs - 1); sb[3] = row(FE, clampF(zs - 1));
+    sa[4] = face(FW, zs);     sb[4] = row(FE, clampF(zs));
+    sa[5] = face(FW, zs - 1); sb[5] = face(FWY, zs - 1);
+    synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+    um = ra[0]; vm = rb[0];
+    u0 = ra[1]; v0 = rb[1];
+    up = ra[2]; vp = rb[2];
+    wm = ra[3]; w0 = ra[4];
+    if (rb[3] < emin || rb[4] < emin) clampflag = 1;
+    em = rb[3] > emin ? rb[3] : emin;
+    e0 = rb[4] > emin ? rb[4] : emin;
+    wxm = ra[5]; wym = rb[5];
+    __syncthreads();
+    const bool da2[3] = {true, true, true};
+    const bool db2[3] = {false, false, false};
+    sa[0] = lvl(FU, zs);  sb[0] = lvl(FUY, zs);
+    sa[1] = lvl(FV, zs);  sb[1] = lvl(FVY, zs);
+    sa[2] = face(FW, zs); sb[2] = face(FWY, zs);
+    synth_pairs<T, NX, 3>(buf, sa, da2, sb, db2, ra, rb, a.kxp, tw, tid);
+    ux0 = ra[0]; uy0 = rb[0];
+    vx0 = ra[1]; vy0 = rb[1];
+    wx0 = ra[2]; wy0 = rb[2];
+    __syncthreads();
+    const float g[9] = {(float)ux0, (float)uy0, (float)((up - um) * todzm), (float)vx0, (float)vy0,
+                        (float)((vp - vm) * todzm), (float)(T(0.5) * (wxm + wx0)),
+                        (float)(T(0.5) * (wym + wy0)), (float)((w0 - wm) * dzm)};
+    ok0 = closure(zs, g, t0);
+  }
+
+  for (int iz = zs; iz < z1; ++iz) {
+    T ex, ey;
+    {
+      T ra[6], rb[6];
+      const cx<T>* sa[6];
+      const cx<T>* sb[6];
+      const bool da[6] = {false, false, true, true, true, true};
+      const bool db[6] = {false, false, false, false, false, false};
+      sa[0] = lvl(FU, iz + 2);  sb[0] = lvl(FV, iz + 2);
+      sa[1] = face(FW, iz + 1); sb[1] = row(FE, clampF(iz + 1));
+      sa[2] = lvl(FU, iz + 1);  sb[2] = lvl(FUY, iz + 1);
+      sa[3] = lvl(FV, iz + 1);  sb[3] = lvl(FVY, iz + 1);
+      sa[4] = face(FW, iz + 1); sb[4] = face(FWY, iz + 1);
+      // ex, ey at face iz; at iz = nnz-1 face nnz-2 again (derivs2.f90:279)
+      const int ie = (iz == nz - 1) ? nz - 2 : iz;
+      sa[5] = (iz >= 1 && iz <= nz - 1) ? row(FE, ie) : nullptr;
+      sb[5] = (iz >= 1 && iz <= nz - 1) ? row(FEY, ie) : nullptr;
+      synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      upp = ra[0]; vpp = rb[0];
+      wp = ra[1];
+      if (rb[1] < emin) clampflag = 1;
+      ep = rb[1] > emin ? rb[1] : emin;
+      ux1 = ra[2]; uy1 = rb[2];
+      vx1 = ra[3]; vy1 = rb[3];
+      wxp = ra[4]; wyp = rb[4];
+      ex = ra[5]; ey = rb[5];
+    }
+    float t1[6];
+    bool ok1;
+    {
+      const float g[9] = {(float)ux1, (float)uy1, (float)((upp - u0) * todzm), (float)vx1, (float)vy1,
+                          (float)((vpp - v0) * todzm), (float)(T(0.5) * (wx0 + wxp)),
+                          (float)(T(0.5) * (wy0 + wyp)), (float)((wp - w0) * dzm)};
+      ok1 = closure(iz + 1, g, t1);
+    }
+    const bool top = (iz == nz);
+    const T xm = xkk_of(em), x0 = xkk_of(e0), xp = xkk_of(ep);
+    // final face-iz stresses (carried to the next level as face iz-1)
+    T tuw0, tvw0;
+    if (top) {                                                    // upperbc.f90:304-305
+      tuw0 = c_hi * (u0 + ugal);
+      tvw0 = c_hi * (v0 + vgal);
+    } else if (ok0 && ok1) {
+      tuw0 = (T)(0.5f * (t0[4] + t1[4]));
+      tvw0 = (T)(0.5f * (t0[5] + t1[5]));
+    } else {                                                      // compsgs1.f90:152-153
+      tuw0 = -x0 * ((up - u0) * dzm + wx0);
+      tvw0 = -x0 * ((vp - v0) * dzm + wy0);
+    }
+    if (iz >= z0) {
+      T fa[6], fb[6];
+      double sums[7];
+      {
+        const T xs = xm + x0;
+        T tuu = -xs * ux0;                                        // compsgs1.f90:144-146
+        T tvv = -xs * vy0;
+        T tuv = -xs * (uy0 + vx0) * T(0.5);
+        const T wz0 = (w0 - wm) * dzm;
+        const T wzp = (wp - w0) * dzm;
+        T tww0 = -xs * wz0;                                       // compsgs1.f90:129,151
+        T twwp = -(x0 + xp) * wzp;
+        if (ok0) {
+          const float tr = (t0[0] + t0[1] + t0[2]) * (1.0f / 3.0f);
+          tuu = (T)(t0[0] - tr);
+          tvv = (T)(t0[1] - tr);
+          tuv = (T)t0[3];
+          tww0 = (T)(t0[2] - tr);
+        }
+        if (ok1) twwp = (T)(t1[2] - (t1[0] + t1[1] + t1[2]) * (1.0f / 3.0f));
+        T tuwm, tvwm;
+        if (iz == 1) {                                            // lowerbc.f90:129-130
+          tuwm = -c_lo * (u0 + ugal);
+          tvwm = -c_lo * (v0 + vgal);
+        } else {
+          tuwm = fuw;
+          tvwm = fvw;
+        }
+        // compadv1.f90:19-51 and compsgs3.f90:205-224
+        fa[0] = -(u0 * u0 + tuu);
+        fb[0] = T(0.016) - ((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
+        fa[1] = -(u0 * v0 + tuv);
+        fb[1] = -((vp + v0) * w0 - (vm + v0) * wm) * todzm - (tvw0 - tvwm) * dzm;
+        fa[3] = -(v0 * v0 + tvv);
+        T Fxw = 0, Fyw = 0, Hw = 0, Fxe = 0, Fye = 0, He = 0;
+        if (!top) {
+          Fxw = -(T(0.5) * (u0 + up) * w0 + tuw0);
+          Fyw = -(T(0.5) * (v0 + vp) * w0 + tvw0);
+          const T wa = T(0.5) * (wp + w0), wb = T(0.5) * (w0 + wm);
+          Hw = -(wa * wa - wb * wb) * dzm - (twwp - tww0) * dzm;
+          Fxe = -T(0.5) * e0 * (u0 + up) + T(2) * x0 * ex;
+          Fye = -T(0.5) * e0 * (v0 + vp) + T(2) * x0 * ey;
+          He = -(wp * ep - wm * em) * todzm
+               + ((xp + x0) * (ep - e0) - (x0 + xm) * (e0 - em)) * dzm2 - ceps * e0 * sqrt(e0);
+        }
+        fa[2] = Fxw; fb[2] = Hw;
+        fb[3] = Fyw;
+        fa[4] = Fxe; fb[4] = He;
+        fa[5] = Fye; fb[5] = T(0);
+        const double dv = (double)(ux0 + vy0 + wz0);              // comp1.f90:145
+        sums[0] = dv * dv;
+        sums[1] = (double)tuu; sums[2] = (double)tvv; sums[3] = (double)tuv;
+        sums[4] = (double)tww0; sums[5] = (double)tuw0; sums[6] = (double)tvw0;
+      }
+      const unsigned long long bal = __ballot(ok0);
+      if (lane == 0) nacc += (unsigned long long)__popcll(bal);
+      {
+        const int nq = mn ? 7 : 1;
+        for (int q = 0; q < nq; ++q) {
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) sums[q] += __shfl_xor(sums[q], off, 64);
+        }
+        if (lane == 0)
+          for (int q = 0; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
+      }
+      {
+        cx<T>* A = a.A;
+        const size_t o = (size_t)iz * plane + rowoff;
+        const size_t as = a.astride;
+        const T* kxp = a.kxp;
+        auto emit = [&](int q, int k, cx<T> fA, cx<T> fB) {
+          const cx<T> ikx = {0, kxp[k]};
+          switch (q) {
+            case 0: A[0 * as + o + k] = cadd(cmul(ikx, fA), fB); break;          // Gu
+            case 1: A[1 * as + o + k] = fA;                                       // Fyu
+                    A[2 * as + o + k] = cadd(cmul(ikx, fA), fB); break;          // Gv
+            case 2: if (!top) A[4 * as + o + k] = cadd(cmul(ikx, fA), fB); break; // Gw
+            case 3: A[3 * as + o + k] = fA;                                       // Fyv
+                    if (!top) A[5 * as + o + k] = fB; break;                     // Fyw
+            case 4: if (!top) A[6 * as + o + k] = cadd(cmul(ikx, fA), fB); break; // Ge
+            default: if (!top) A[7 * as + o + k] = fA; break;                    // Fye
+          }
+        };
+        anal_pairs<T, NX, 6>(buf, fa, fb, tw, tid, emit);
+      }
+      {
+        const int nq = mn ? 7 : 1;
+        if (tid < nq) {
+          double sq = 0;
+          for (int w = 0; w < F::NW; ++w) sq += red[tid * F::NW + w];
+          a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sq;
+        }
+      }
+    } else {
+      __syncthreads();                     // every thread has read the synthesis results
+    }
+    // slide the window
+    um = u0; u0 = up; up = upp;
+    vm = v0; v0 = vp; vp = vpp;
+    wm = w0; w0 = wp;
+    em = e0; e0 = ep;
+    ux0 = ux1; uy0 = uy1; vx0 = vx1; vy0 = vy1;
+    wxm = wx0; wym = wy0; wx0 = wxp; wy0 = wyp;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t0[q] = t1[q];
+    ok0 = ok1;
+    fuw = tuw0; fvw = tvw0;
+  }
+  if (clampflag) atomicOr(a.flags, FLAG_ECLAMP);
+  if (lane == 0) {
+    atomicAdd(counts, nacc);
+    atomicAdd(counts + 1, 64ull * (unsigned long long)(z1 - z0));
+  }
 }
 
 // ---------------------------------------------------------------------------

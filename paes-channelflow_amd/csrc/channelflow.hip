@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -77,6 +78,14 @@ struct cf_ctx {
   cf::WallScalars* ws = nullptr;
   std::vector<double> sgs_host;  // 6*nz, valid after a mnout stage 1
   bool have_sgs = false;
+  // learned closure (k_rows_mlp in place of k_rows when set)
+  cf::mlp::Net* closure_net = nullptr;
+  unsigned long long* closure_counts = nullptr;   // accepted, evaluated cells
+  bool closure_on = false;
+  // box-filter work buffers (allocated on first use)
+  double* filt_in = nullptr;    // u, v, w, e physical, 4 * nz*ny*nx
+  double* filt_out = nullptr;
+  size_t filt_out_n = 0;
 };
 
 namespace {
@@ -124,6 +133,8 @@ struct Engine {
                         const double* e) = 0;
   // which: 0 u, 1 v, 2 w, 3 e, 4 p, 5 ru, 6 rv, 7 rw, 8 re, 9 run, 10 rvn, 11 rwn
   virtual int get_field(cf_ctx* ctx, int which, double* out) = 0;
+  // the same field, physical, left in ctx->staging (enqueued on the stream)
+  virtual int physical(cf_ctx* ctx, int which) = 0;
 };
 
 template <typename T, int NX>
@@ -197,6 +208,7 @@ struct EngineT : Engine {
     c.todzm = 0.5 / dz;
     const double dsl = std::pow(dx * dy * dz, 1.0 / 3.0);                // les.F90:118
     c.alk = p.alk > 0 ? p.alk : dsl;
+    c.dsl = dsl;
     c.xkmax = dz * dz / ctx->dt / 5.0;                                  // les.F90:216
     c.ceps = 0.93 / dsl;                                                // les.F90:120
     c.emin = 1.0e-6;                                                    // les.F90:102
@@ -261,7 +273,15 @@ struct EngineT : Engine {
       a.flags = &ctx->ws->flags;
       a.c = c;
       const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
-      hipLaunchKernelGGL((cf::k_rows<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+      bool learned = false;
+      if constexpr ((3 * NX / 2) % 64 == 0) {
+        if (ctx->closure_on) {
+          hipLaunchKernelGGL((cf::k_rows_mlp<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a,
+                             ctx->closure_net, ctx->closure_counts);
+          learned = true;
+        }
+      }
+      if (!learned) hipLaunchKernelGGL((cf::k_rows<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
     }
     mark();
     {
@@ -399,6 +419,15 @@ struct EngineT : Engine {
   }
 
   int get_field(cf_ctx* ctx, int which, double* out) override {
+    const size_t n = (size_t)ctx->nz * NY * NX;
+    int rc = physical(ctx, which);
+    if (rc != CF_OK) return rc;
+    HIPCHK(hipMemcpyAsync(out, ctx->staging, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CF_OK;
+  }
+
+  int physical(cf_ctx* ctx, int which) override {
     const int nz = ctx->nz;
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)nz * NY * NX;
@@ -415,10 +444,7 @@ struct EngineT : Engine {
     } else {
       rc = from_band<T>(ctx, cptr(bands[which]), cptr(ctx->xs_tmp), cptr(ctx->twNX));
     }
-    if (rc != CF_OK) return rc;
-    HIPCHK(hipMemcpyAsync(out, ctx->staging, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    return CF_OK;
+    return rc;
   }
 };
 
@@ -764,6 +790,487 @@ int64_t cf_device_bytes(const cf_ctx* ctx) { return ctx ? ctx->bytes : 0; }
 
 const char* cf_last_error(const cf_ctx* ctx) {
   return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned SGS closure: the network object (trainer and published tables),
+// the closure switch of a context, and the training-data box filter.  No
+// counterpart in the reference.
+// ---------------------------------------------------------------------------
+namespace mlpns = cf::mlp;
+
+struct cf_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<void*> allocs;
+  float in_mu[mlpns::NIN] = {}, in_sd[mlpns::NIN] = {}, out_mu[mlpns::NOUT] = {},
+        out_sd[mlpns::NOUT] = {};
+  mlpns::Net* net = nullptr;      // published forward tables + normalisation
+  mlpns::bf16x8* bw = nullptr;    // published backward tables
+  float* w = nullptr;             // FP32 master weights
+  float* m = nullptr;             // Adam moments
+  float* v = nullptr;
+  float* grad = nullptr;
+  float* partial = nullptr;       // [kMaxBlk][NPARAM+1]
+  float* losses = nullptr;        // [kMaxSteps] per-step loss of one train call
+  float* x = nullptr;             // samples [11][cap], normalised
+  float* t = nullptr;             // targets [6][cap], normalised
+  int64_t cap = 0, n = 0;
+  int64_t step = 0;               // Adam step count
+  float nrm_host[48] = {};        // staging of the Net normalisation block
+};
+
+namespace {
+
+constexpr int kMaxBlk = 1024;
+constexpr int kMaxSteps = 1 << 16;
+
+#define MHIPCHK(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      m->err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+      return CF_E_HIP;                                                                 \
+    }                                                                                  \
+  } while (0)
+
+template <typename X>
+int malloc_mlp(cf_mlp* m, X** p, size_t count) {
+  void* q = nullptr;
+  MHIPCHK(hipMalloc(&q, count * sizeof(X)));
+  MHIPCHK(hipMemset(q, 0, count * sizeof(X)));
+  m->allocs.push_back(q);
+  *p = (X*)q;
+  return CF_OK;
+}
+
+// Upload the normalisation into the published Net and rebuild its tables.
+int publish(cf_mlp* m) {
+  // Net::in_mu[16], in_rs[16], out_sd[8], out_mu[8] are contiguous
+  static_assert(offsetof(mlpns::Net, out_mu) - offsetof(mlpns::Net, in_mu) == 40 * sizeof(float),
+                "normalisation block layout");
+  float* nrm = m->nrm_host;
+  std::memset(nrm, 0, sizeof(m->nrm_host));
+  for (int k = 0; k < mlpns::NIN; ++k) {
+    nrm[k] = m->in_mu[k];
+    nrm[16 + k] = 1.0f / m->in_sd[k];
+  }
+  for (int k = 0; k < mlpns::NOUT; ++k) {
+    nrm[32 + k] = m->out_sd[k];
+    nrm[40 + k] = m->out_mu[k];
+  }
+  MHIPCHK(hipMemcpyAsync(m->net->in_mu, nrm, sizeof(m->nrm_host), hipMemcpyHostToDevice, m->stream));
+  const int tot = mlpns::FTN * 512 + mlpns::BTN * 512 + mlpns::FBN;
+  hipLaunchKernelGGL(mlpns::k_mlp_publish, dim3((tot + 255) / 256), dim3(256), 0, m->stream, m->w,
+                     m->net, m->bw);
+  MHIPCHK(hipGetLastError());
+  return CF_OK;
+}
+
+int check_norm(cf_mlp* m) {
+  for (int k = 0; k < mlpns::NIN; ++k)
+    if (!(m->in_sd[k] > 0) || !std::isfinite(m->in_mu[k])) {
+      m->err = "input normalisation: sd must be > 0 (set with cf_mlp_set_norm)";
+      return CF_E_ARG;
+    }
+  for (int k = 0; k < mlpns::NOUT; ++k)
+    if (!(m->out_sd[k] > 0) || !std::isfinite(m->out_mu[k])) {
+      m->err = "output normalisation: sd must be > 0";
+      return CF_E_ARG;
+    }
+  return CF_OK;
+}
+
+// Gradient over samples [off, off+cnt) in order (cnt > 0) or over batch
+// random samples (cnt == 0); then Adam when update != 0.
+int grad_step(cf_mlp* m, int64_t off, int64_t cnt, int64_t batch, uint64_t seed, float lr,
+              int update, float* loss_dev) {
+  const int64_t total = cnt > 0 ? cnt : batch;
+  const int64_t nchunk = (total + 63) / 64;
+  const int nblk = (int)std::min<int64_t>(kMaxBlk, (nchunk + mlpns::TW - 1) / mlpns::TW);
+  mlpns::TrainArgs ta{};
+  ta.net = m->net;
+  ta.bw = m->bw;
+  ta.x = m->x;
+  ta.t = m->t;
+  ta.cap = m->cap;
+  ta.n = m->n;
+  ta.off = off;
+  ta.cnt = cnt;
+  ta.batch = batch;
+  ta.seed = seed;
+  ta.gscale = (float)(2.0 / ((double)total * mlpns::NOUT));
+  ta.partial = m->partial;
+  hipLaunchKernelGGL(mlpns::k_mlp_grad, dim3(nblk), dim3(64 * mlpns::TW), 0, m->stream, ta);
+  mlpns::AdamArgs aa{};
+  aa.partial = m->partial;
+  aa.nblk = nblk;
+  aa.w = m->w;
+  aa.m = m->m;
+  aa.v = m->v;
+  aa.grad = m->grad;
+  aa.loss = loss_dev;
+  aa.lr = lr;
+  aa.b1 = 0.9f;
+  aa.b2 = 0.999f;
+  aa.eps = 1e-8f;
+  if (update) {
+    m->step += 1;
+    aa.bc1 = (float)(1.0 - std::pow(0.9, (double)m->step));
+    aa.bc2 = (float)(1.0 - std::pow(0.999, (double)m->step));
+  } else {
+    aa.bc1 = aa.bc2 = 1.f;
+  }
+  aa.inv_n6 = (float)(1.0 / ((double)total * mlpns::NOUT));
+  aa.update = update;
+  hipLaunchKernelGGL(mlpns::k_mlp_adam, dim3((mlpns::NPARAM + 1 + 255) / 256), dim3(256), 0,
+                     m->stream, aa);
+  MHIPCHK(hipGetLastError());
+  if (update) return publish(m);
+  return CF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cf_mlp_create(int32_t device, cf_mlp** out) {
+  if (!out) return CF_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    g_create_error = "no HIP device";
+    return CF_E_NODEV;
+  }
+  cf_mlp* m = new cf_mlp();
+  m->device = device;
+  int rc = CF_OK;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
+    g_create_error = "hipSetDevice/hipStreamCreate failed";
+    delete m;
+    return CF_E_HIP;
+  }
+  if ((rc = malloc_mlp(m, &m->net, 1)) != CF_OK || (rc = malloc_mlp(m, &m->bw, mlpns::BTN * 64)) ||
+      (rc = malloc_mlp(m, &m->w, mlpns::NPARAM)) || (rc = malloc_mlp(m, &m->m, mlpns::NPARAM)) ||
+      (rc = malloc_mlp(m, &m->v, mlpns::NPARAM)) || (rc = malloc_mlp(m, &m->grad, mlpns::NPARAM)) ||
+      (rc = malloc_mlp(m, &m->partial, (size_t)kMaxBlk * (mlpns::NPARAM + 1))) ||
+      (rc = malloc_mlp(m, &m->losses, kMaxSteps))) {
+    g_create_error = m->err;
+    cf_mlp_destroy(m);
+    return rc;
+  }
+  for (int k = 0; k < mlpns::NIN; ++k) m->in_sd[k] = 1.f;
+  for (int k = 0; k < mlpns::NOUT; ++k) m->out_sd[k] = 1.f;
+  if ((rc = publish(m)) != CF_OK || hipStreamSynchronize(m->stream) != hipSuccess) {
+    g_create_error = m->err;
+    cf_mlp_destroy(m);
+    return rc != CF_OK ? rc : CF_E_HIP;
+  }
+  *out = m;
+  return CF_OK;
+}
+
+int cf_mlp_destroy(cf_mlp* m) {
+  if (!m) return CF_OK;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (void* p : m->allocs) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return CF_OK;
+}
+
+int cf_mlp_set_params(cf_mlp* m, const float* params) {
+  if (!m || !params) return CF_E_ARG;
+  for (int i = 0; i < mlpns::NPARAM; ++i)
+    if (!std::isfinite(params[i])) {
+      m->err = "non-finite parameter";
+      return CF_E_ARG;
+    }
+  MHIPCHK(hipSetDevice(m->device));
+  MHIPCHK(hipMemcpyAsync(m->w, params, mlpns::NPARAM * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  int rc = publish(m);
+  if (rc != CF_OK) return rc;
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  return CF_OK;
+}
+
+int cf_mlp_get_params(cf_mlp* m, float* params) {
+  if (!m || !params) return CF_E_ARG;
+  MHIPCHK(hipSetDevice(m->device));
+  MHIPCHK(hipMemcpyAsync(params, m->w, mlpns::NPARAM * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  return CF_OK;
+}
+
+int cf_mlp_set_norm(cf_mlp* m, const float* in_mu, const float* in_sd, const float* out_mu,
+                    const float* out_sd) {
+  if (!m || !in_mu || !in_sd || !out_mu || !out_sd) return CF_E_ARG;
+  std::memcpy(m->in_mu, in_mu, sizeof(m->in_mu));
+  std::memcpy(m->in_sd, in_sd, sizeof(m->in_sd));
+  std::memcpy(m->out_mu, out_mu, sizeof(m->out_mu));
+  std::memcpy(m->out_sd, out_sd, sizeof(m->out_sd));
+  int rc = check_norm(m);
+  if (rc != CF_OK) return rc;
+  MHIPCHK(hipSetDevice(m->device));
+  rc = publish(m);
+  if (rc != CF_OK) return rc;
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  return CF_OK;
+}
+
+int cf_mlp_get_norm(cf_mlp* m, float* in_mu, float* in_sd, float* out_mu, float* out_sd) {
+  if (!m) return CF_E_ARG;
+  if (in_mu) std::memcpy(in_mu, m->in_mu, sizeof(m->in_mu));
+  if (in_sd) std::memcpy(in_sd, m->in_sd, sizeof(m->in_sd));
+  if (out_mu) std::memcpy(out_mu, m->out_mu, sizeof(m->out_mu));
+  if (out_sd) std::memcpy(out_sd, m->out_sd, sizeof(m->out_sd));
+  return CF_OK;
+}
+
+// File: "CFMLP001", int32 {11, 64, 6, 9478}, float in_mu[11], in_sd[11],
+// out_mu[6], out_sd[6], params[9478], little endian.
+int cf_mlp_save(cf_mlp* m, const char* path) {
+  if (!m || !path) return CF_E_ARG;
+  std::vector<float> p(mlpns::NPARAM);
+  int rc = cf_mlp_get_params(m, p.data());
+  if (rc != CF_OK) return rc;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    m->err = std::string("cannot open ") + path;
+    return CF_E_ARG;
+  }
+  const int32_t hdr[4] = {mlpns::NIN, mlpns::NH, mlpns::NOUT, mlpns::NPARAM};
+  bool ok = std::fwrite("CFMLP001", 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
+            std::fwrite(m->in_mu, 4, mlpns::NIN, f) == (size_t)mlpns::NIN &&
+            std::fwrite(m->in_sd, 4, mlpns::NIN, f) == (size_t)mlpns::NIN &&
+            std::fwrite(m->out_mu, 4, mlpns::NOUT, f) == (size_t)mlpns::NOUT &&
+            std::fwrite(m->out_sd, 4, mlpns::NOUT, f) == (size_t)mlpns::NOUT &&
+            std::fwrite(p.data(), 4, p.size(), f) == p.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) {
+    m->err = std::string("write failed: ") + path;
+    return CF_E_ARG;
+  }
+  return CF_OK;
+}
+
+int cf_mlp_load(cf_mlp* m, const char* path) {
+  if (!m || !path) return CF_E_ARG;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) {
+    m->err = std::string("cannot open ") + path;
+    return CF_E_ARG;
+  }
+  char magic[8];
+  int32_t hdr[4];
+  float nrm[2 * mlpns::NIN + 2 * mlpns::NOUT];
+  std::vector<float> p(mlpns::NPARAM);
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "CFMLP001", 8) == 0 &&
+            std::fread(hdr, 4, 4, f) == 4 && hdr[0] == mlpns::NIN && hdr[1] == mlpns::NH &&
+            hdr[2] == mlpns::NOUT && hdr[3] == mlpns::NPARAM &&
+            std::fread(nrm, 4, 34, f) == 34 && std::fread(p.data(), 4, p.size(), f) == p.size();
+  std::fclose(f);
+  if (!ok) {
+    m->err = std::string("not a CFMLP001 network file: ") + path;
+    return CF_E_ARG;
+  }
+  int rc = cf_mlp_set_norm(m, nrm, nrm + mlpns::NIN, nrm + 2 * mlpns::NIN,
+                           nrm + 2 * mlpns::NIN + mlpns::NOUT);
+  if (rc != CF_OK) return rc;
+  return cf_mlp_set_params(m, p.data());
+}
+
+int cf_mlp_forward(cf_mlp* m, int64_t n, const float* feats, float* stresses) {
+  if (!m || n < 0 || (n > 0 && (!feats || !stresses))) return CF_E_ARG;
+  if (n == 0) return CF_OK;
+  MHIPCHK(hipSetDevice(m->device));
+  float *df = nullptr, *dt = nullptr;
+  MHIPCHK(hipMallocAsync((void**)&df, (size_t)n * mlpns::NIN * sizeof(float), m->stream));
+  MHIPCHK(hipMallocAsync((void**)&dt, (size_t)n * mlpns::NOUT * sizeof(float), m->stream));
+  MHIPCHK(hipMemcpyAsync(df, feats, (size_t)n * mlpns::NIN * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(mlpns::k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, m->stream,
+                     m->net, df, dt, n);
+  MHIPCHK(hipGetLastError());
+  MHIPCHK(hipMemcpyAsync(stresses, dt, (size_t)n * mlpns::NOUT * sizeof(float), hipMemcpyDeviceToHost,
+                         m->stream));
+  MHIPCHK(hipFreeAsync(df, m->stream));
+  MHIPCHK(hipFreeAsync(dt, m->stream));
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  return CF_OK;
+}
+
+int cf_mlp_reserve(cf_mlp* m, int64_t capacity) {
+  if (!m || capacity <= 0) return CF_E_ARG;
+  MHIPCHK(hipSetDevice(m->device));
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  for (float** p : {&m->x, &m->t}) {
+    if (*p) {
+      auto it = std::find(m->allocs.begin(), m->allocs.end(), (void*)*p);
+      if (it != m->allocs.end()) m->allocs.erase(it);
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  }
+  m->cap = 0;
+  m->n = 0;
+  int rc;
+  if ((rc = malloc_mlp(m, &m->x, (size_t)capacity * mlpns::NIN)) != CF_OK) return rc;
+  if ((rc = malloc_mlp(m, &m->t, (size_t)capacity * mlpns::NOUT)) != CF_OK) return rc;
+  m->cap = capacity;
+  return CF_OK;
+}
+
+int cf_mlp_put_samples(cf_mlp* m, int64_t offset, int64_t n, const float* feats,
+                       const float* targets) {
+  if (!m || offset < 0 || n <= 0 || !feats || !targets || offset + n > m->cap) {
+    if (m) m->err = "samples outside the reserved capacity";
+    return CF_E_ARG;
+  }
+  int rc = check_norm(m);
+  if (rc != CF_OK) return rc;
+  MHIPCHK(hipSetDevice(m->device));
+  std::vector<float> col((size_t)n);
+  for (int k = 0; k < mlpns::NIN; ++k) {
+    const float mu = m->in_mu[k], rs = 1.0f / m->in_sd[k];
+    for (int64_t i = 0; i < n; ++i) col[(size_t)i] = (feats[i * mlpns::NIN + k] - mu) * rs;
+    MHIPCHK(hipMemcpy(m->x + (size_t)k * m->cap + offset, col.data(), (size_t)n * sizeof(float),
+                      hipMemcpyHostToDevice));
+  }
+  for (int k = 0; k < mlpns::NOUT; ++k) {
+    const float mu = m->out_mu[k], rs = 1.0f / m->out_sd[k];
+    for (int64_t i = 0; i < n; ++i) col[(size_t)i] = (targets[i * mlpns::NOUT + k] - mu) * rs;
+    MHIPCHK(hipMemcpy(m->t + (size_t)k * m->cap + offset, col.data(), (size_t)n * sizeof(float),
+                      hipMemcpyHostToDevice));
+  }
+  m->n = std::max(m->n, offset + n);
+  return CF_OK;
+}
+
+int cf_mlp_loss_grad(cf_mlp* m, int64_t offset, int64_t n, float* loss, float* grad) {
+  if (!m || offset < 0 || n <= 0 || offset + n > m->n) {
+    if (m) m->err = "sample range outside the stored samples";
+    return CF_E_ARG;
+  }
+  MHIPCHK(hipSetDevice(m->device));
+  int rc = grad_step(m, offset, n, 0, 0, 0.f, 0, m->losses);
+  if (rc != CF_OK) return rc;
+  if (grad)
+    MHIPCHK(hipMemcpyAsync(grad, m->grad, mlpns::NPARAM * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  if (loss) MHIPCHK(hipMemcpyAsync(loss, m->losses, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  return CF_OK;
+}
+
+int cf_mlp_train(cf_mlp* m, int32_t nsteps, int64_t batch, float lr, uint64_t seed, float* losses) {
+  if (!m || nsteps <= 0 || nsteps > kMaxSteps || batch < 0 || !(lr > 0) || m->n <= 0) {
+    if (m) m->err = "cf_mlp_train: bad arguments or no samples";
+    return CF_E_ARG;
+  }
+  MHIPCHK(hipSetDevice(m->device));
+  for (int s = 0; s < nsteps; ++s) {
+    const uint64_t sd = seed * 0x9E3779B97F4A7C15ull + (uint64_t)(m->step + 1) * 0xD1B54A32D192ED03ull;
+    int rc = batch > 0 ? grad_step(m, 0, 0, batch, sd, lr, 1, m->losses + s)
+                       : grad_step(m, 0, m->n, 0, 0, lr, 1, m->losses + s);
+    if (rc != CF_OK) return rc;
+  }
+  if (losses)
+    MHIPCHK(hipMemcpyAsync(losses, m->losses, (size_t)nsteps * sizeof(float), hipMemcpyDeviceToHost,
+                           m->stream));
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  return CF_OK;
+}
+
+int cf_mlp_reset_optimizer(cf_mlp* m) {
+  if (!m) return CF_E_ARG;
+  MHIPCHK(hipSetDevice(m->device));
+  MHIPCHK(hipMemsetAsync(m->m, 0, mlpns::NPARAM * sizeof(float), m->stream));
+  MHIPCHK(hipMemsetAsync(m->v, 0, mlpns::NPARAM * sizeof(float), m->stream));
+  MHIPCHK(hipStreamSynchronize(m->stream));
+  m->step = 0;
+  return CF_OK;
+}
+
+const char* cf_mlp_last_error(const cf_mlp* m) { return m ? m->err.c_str() : g_create_error.c_str(); }
+
+int cf_set_closure(cf_ctx* ctx, const cf_mlp* m) {
+  if (!ctx) return CF_E_ARG;
+  HIPCHK(hipSetDevice(ctx->p.device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (!m) {
+    ctx->closure_on = false;
+    return CF_OK;
+  }
+  if ((ctx->nxb % 64) != 0) {
+    ctx->err = "the learned closure needs nnx in {128, 256, 512} (3/2-grid rows of whole wavefronts)";
+    return CF_E_ARG;
+  }
+  if (m->device != ctx->p.device) {
+    ctx->err = "network and context are on different devices";
+    return CF_E_ARG;
+  }
+  if (!ctx->closure_net) {
+    int rc = dalloc(ctx, &ctx->closure_net, 1);
+    if (rc != CF_OK) return rc;
+    rc = dalloc(ctx, &ctx->closure_counts, 2);
+    if (rc != CF_OK) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpyAsync(ctx->closure_net, m->net, sizeof(mlpns::Net), hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->closure_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->closure_on = true;
+  return CF_OK;
+}
+
+int cf_get_closure_counts(cf_ctx* ctx, int64_t* counts) {
+  if (!ctx || !counts) return CF_E_ARG;
+  counts[0] = counts[1] = 0;
+  if (!ctx->closure_counts) return CF_OK;
+  unsigned long long h[2];
+  HIPCHK(hipMemcpyAsync(h, ctx->closure_counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->closure_counts, 0, sizeof(h), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  counts[0] = (int64_t)h[0];
+  counts[1] = (int64_t)h[1];
+  return CF_OK;
+}
+
+int cf_filter_snapshot(cf_ctx* ctx, int32_t fx, int32_t fy, int32_t fz, double* out) {
+  if (!ctx || !out) return CF_E_ARG;
+  auto okf = [](int f) { return f == 1 || f == 2 || f == 4 || f == 8; };
+  if (!okf(fx) || !okf(fy) || !(fz == 1 || fz == 2 || fz == 4) || ctx->nx % fx || ctx->ny % fy ||
+      ctx->nz % fz) {
+    ctx->err = "filter widths: fx, fy in {1,2,4,8}, fz in {1,2,4}, dividing the grid";
+    return CF_E_ARG;
+  }
+  HIPCHK(hipSetDevice(ctx->p.device));
+  const size_t n = (size_t)ctx->nz * ctx->ny * ctx->nx;
+  const size_t nc = n / ((size_t)fx * fy * fz);
+  if (!ctx->filt_in) {
+    int rc = dalloc(ctx, &ctx->filt_in, 4 * n);
+    if (rc != CF_OK) return rc;
+  }
+  if (ctx->filt_out_n < 11 * nc) {
+    int rc = dalloc(ctx, &ctx->filt_out, 11 * nc);
+    if (rc != CF_OK) return rc;
+    ctx->filt_out_n = 11 * nc;
+  }
+  for (int f = 0; f < 4; ++f) {
+    int rc = ctx->eng->physical(ctx, f);
+    if (rc != CF_OK) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->filt_in + f * n, ctx->staging, n * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  mlpns::FilterArgs fa{ctx->filt_in, ctx->filt_in + n, ctx->filt_in + 2 * n, ctx->filt_in + 3 * n,
+                       ctx->filt_out, ctx->nx, ctx->ny, ctx->nz, fx, fy, fz};
+  hipLaunchKernelGGL(mlpns::k_box_filter, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream, fa);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, ctx->filt_out, 11 * nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CF_OK;
 }
 
 }  // extern "C"
