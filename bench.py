@@ -3,11 +3,15 @@
 
 Metric: grid-point RK3 steps per second (nnx*nny*nnz*steps / time) on the
 256x256x128 channel at Re_tau = 395 (z0 = exp(-0.4*5)/395), seeded log-law +
-0.1 u_tau noise, TKE SGS closure.  ``value`` is the FP32 path (FP32 fields,
-transforms, products and SGS; FP64 Poisson RHS, tridiagonal solve, projection
-and update); the FP64 plain path is timed beside it.  Fields are resident in
-HBM before the timed region; the timed region is K full steps on the library
-stream bracketed by stream synchronisation, measured with HIP events.
+0.1 u_tau noise.  ``value`` is the FP32 path with the learned SGS closure:
+- FP32 fields, transforms, products and SGS;
+- the BF16-MFMA network at every 3/2-grid cell, with TKE fallback;
+- FP64 Poisson RHS, tridiagonal solve, projection and update.
+The network is the trained one shipped in paes-channelflow_amd/weights/.
+Timed beside it: the FP64 plain path (TKE closure) and the FP32 path with
+the TKE closure.  Fields are resident in HBM before the timed region; the
+timed region is K full steps on the library stream bracketed by stream
+synchronisation, measured with HIP events.
 
 One JSON line is printed.  ``roofline`` prices the dominant kernel (the
 row-tendency kernel k_rows): the bytes its launch must move -- the 8 y-synthesised
@@ -31,6 +35,7 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "paes-channelflow_amd"))
+WEIGHTS = os.path.join(ROOT, "paes-channelflow_amd", "weights", "sgs_mlp.cfmlp")
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_GPSTEP = {"fp32": 436.0, "fp64": 776.0}
@@ -73,6 +78,8 @@ def parse():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--re-tau", type=float, default=395.0)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain path")
+    ap.add_argument("--no-tke32", action="store_true", help="skip the FP32 TKE-closure path")
+    ap.add_argument("--weights", default=WEIGHTS, help="learned closure network file")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=0,
                     help="levels of the CPU-baseline sample (default: the full nz)")
@@ -81,7 +88,7 @@ def parse():
     return ap.parse_args()
 
 
-def run_gpu(args, precision_name):
+def run_gpu(args, precision_name, learned=False):
     import numpy as np
     import channelflow as C
     nx, nz = args.nx, args.nz
@@ -94,6 +101,10 @@ def run_gpu(args, precision_name):
     g = C.ChannelFlow(rp)
     u, v, w, e = seeded_fields(nx, nz, z0)
     g.set_state(u, v, w, e)
+    net = None
+    if learned:
+        net = C.SgsNet(0, args.weights)
+        g.set_closure(net)
     if args.warmup > 0:
         g.step(args.warmup)
     g.sync()
@@ -104,9 +115,12 @@ def run_gpu(args, precision_name):
     g.sync()
     st = g.get_state()
     finite = all(bool(np.isfinite(st[k]).all()) for k in "uvwe")
+    acc = g.closure_counts() if learned else (0, 0)
     out = dict(ms=ms, wall=wall, per_kernel=per_kernel, dt=dt, z0=z0, finite=finite,
-               device_bytes=g.device_bytes())
+               device_bytes=g.device_bytes(), accepted=acc)
     g.close()
+    if net is not None:
+        net.close()
     return out
 
 
@@ -161,11 +175,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print("[bench] copy probe: 256 MiB read + 256 MiB written", file=sys.stderr, flush=True)
-    print(f"[bench] fp32 path {args.nx}x{args.nx}x{args.nz}", file=sys.stderr, flush=True)
-    r32 = run_gpu(args, "fp32")
+    print(f"[bench] fp32 + learned closure {args.nx}x{args.nx}x{args.nz}", file=sys.stderr, flush=True)
+    r32 = run_gpu(args, "fp32", learned=True)
     value = pts * args.steps / (r32["ms"] * 1e-3)
-    print("[bench] fp64 path", file=sys.stderr, flush=True)
+    print("[bench] fp64 plain path", file=sys.stderr, flush=True)
     r64 = None if args.no_fp64 else run_gpu(args, "fp64")
+    print("[bench] fp32 TKE path", file=sys.stderr, flush=True)
+    rt = None if args.no_tke32 else run_gpu(args, "fp32")
     kern = r32["per_kernel"]
     dom = max((k for k in kern if not k.startswith("(")), key=lambda k: kern[k])
     step_ms = r32["ms"] / args.steps
@@ -174,9 +190,12 @@ def main():
     achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
     per_kernel_gbs = {k: kernel_bytes(k, pts, 4) / (t * 1e-3) / 1e9
                       for k, t in kern.items() if kernel_bytes(k, pts, 4) and t > 0}
-    traffic = MEASURED_TRAFFIC.get((dom, args.nx, args.nz, "fp32"))
+    # the learned path runs the row kernel with the network inside (k_rows_mlp):
+    # same row inputs and outputs, so the same algorithmic bytes
+    dom_label = "k_rows_mlp" if dom == "k_rows" else dom
+    traffic = MEASURED_TRAFFIC.get((dom_label, args.nx, args.nz, "fp32"))
     line = {
-        "metric": "grid-pt RK3 steps/s @256^2x128: FP32(+FP64 pressure) TKE-SGS path; % HBM peak",
+        "metric": "grid-pt RK3 steps/s @256^2x128: FP32+learned SGS vs FP64 plain; % HBM peak",
         "value": value,
         "unit": "grid-pt RK3 steps/s",
         "n_gpus": 1,
@@ -186,15 +205,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (fp64 pressure/projection)",
+        "dtype": "fp32 (fp64 pressure/projection; bf16 MFMA closure)",
         "data": "synthetic: seeded log-law + 0.1 u_tau uniform noise, e = 0.01 u_tau^2",
         "config": {
             "workload": f"channel {args.nx}x{args.nx}x{args.nz}, 2pi x pi x 2, Re_tau={args.re_tau:g} "
-                        f"(z0={r32['z0']:.4e}), TKE SGS (imodel 3), Schumann wall (isurface 1), "
+                        f"(z0={r32['z0']:.4e}), learned SGS closure (MLP 11-64-64-64-6, BF16 MFMA) "
+                        f"with TKE fallback (imodel 3), Schumann wall (isurface 1), "
                         f"3/2 dealiasing, dt={r32['dt']:.3e}",
             "grid": [args.nx, args.nx, args.nz],
-            "precision": "fp32 fields/FFT/SGS, fp64 pressure",
-            "learned_sgs": False,
+            "precision": "fp32 fields/FFT/SGS, fp64 pressure, bf16 MFMA network with fp32 accumulation",
+            "learned_sgs": True,
+            "closure_cells_accepted": r32["accepted"][0] / max(r32["accepted"][1], 1),
         },
         "roofline": {
             "bound": "hbm",
@@ -203,10 +224,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": dom,
+            "kernel": dom_label,
             "bytes_per_launch": dom_bytes,
             "ms_per_launch": kern[dom],
-            "basis": f"{dom}: 24 x N x 4 B per launch (3/2-grid x-spectral rows in and out, "
+            "basis": f"{dom_label}: 24 x N x 4 B per launch (3/2-grid x-spectral rows in and out, "
                      "once each) / mean launch time from HIP events on the library stream",
         },
         "step_roofline": {
@@ -217,14 +238,19 @@ def main():
         "kernel_ms_per_stage": kern,
         "kernel_gbs": per_kernel_gbs,
         "fp64_plain": None,
+        "fp32_tke": None,
         "finite": r32["finite"],
     }
+    if rt is not None:
+        vt = pts * args.steps / (rt["ms"] * 1e-3)
+        line["fp32_tke"] = {"value": vt, "ms_per_step": rt["ms"] / args.steps,
+                            "kernel_ms_per_stage": rt["per_kernel"], "finite": rt["finite"]}
     if r64 is not None:
         v64 = pts * args.steps / (r64["ms"] * 1e-3)
         line["fp64_plain"] = {
             "value": v64, "ms_per_step": r64["ms"] / args.steps,
             "hbm_frac": BYTES_PER_GPSTEP["fp64"] * v64 / 1e9 / HBM_PEAK_GBS,
-            "fp32_over_fp64": value / v64, "kernel_ms_per_stage": r64["per_kernel"],
+            "learned_fp32_over_fp64": value / v64, "kernel_ms_per_stage": r64["per_kernel"],
             "finite": r64["finite"],
         }
     if not args.no_cpu_baseline:

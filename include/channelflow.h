@@ -216,9 +216,14 @@ int cf_get_closure_counts(cf_ctx* ctx, int64_t* counts);
 
 /* Training data: box filter of the current state by fx x fy x fz fine cells
  * (fx, fy in {1,2,4,8}, fz in {1,2,4}, dividing the grid) into
- * out[11][ncz][ncy][ncx] doubles: u, v, w (cell centre), w (coarse face
- * K+1, trapezoid over fz+1 fine faces, 0 at the top wall), e (cell centre),
- * uu, vv, ww, uv, uw, vw box means of the cell-centred fine velocities. */
+ * out[17][ncz][ncy][ncx] doubles, all box means:
+ * - u, v, w (cell centre);
+ * - w at the coarse face K+1 (trapezoid over fz+1 fine faces, 0 at the top
+ *   wall);
+ * - e (cell centre);
+ * - uu, vv, ww, uv, uw, vw of the cell-centred fine velocities;
+ * - the fine grid's own TKE-closure stresses t11 t22 t33 t12 t13 t23 at cell
+ *   centres (compsgs1.f90:144-153; face stresses averaged to the centre). */
 int cf_filter_snapshot(cf_ctx* ctx, int32_t fx, int32_t fy, int32_t fz, double* out);
 
 #ifdef __cplusplus

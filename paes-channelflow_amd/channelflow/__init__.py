@@ -281,15 +281,17 @@ class ChannelFlow:
 
     def filter_snapshot(self, fx: int, fy: int, fz: int):
         """Box filter of the current state onto the (nnx/fx, nny/fy, nnz/fz)
-        grid: dict of u, v, wc, wf, e, uu, vv, ww, uv, uw, vw arrays (ncz, ncy, ncx)."""
+        grid: dict of u, v, wc, wf, e, uu, vv, ww, uv, uw, vw and the fine-grid
+        SGS stresses s11 s22 s33 s12 s13 s23, arrays (ncz, ncy, ncx)."""
         nz, ny, nx = self.shape
         shp = (nz // fz, ny // fy, nx // fx)
-        out = np.empty((11,) + shp)
+        out = np.empty((len(FILTER_NAMES),) + shp)
         self._check(self.lib.cf_filter_snapshot(self._h, fx, fy, fz, _ptr(out)))
         return dict(zip(FILTER_NAMES, out))
 
 
-FILTER_NAMES = ("u", "v", "wc", "wf", "e", "uu", "vv", "ww", "uv", "uw", "vw")
+FILTER_NAMES = ("u", "v", "wc", "wf", "e", "uu", "vv", "ww", "uv", "uw", "vw",
+                "s11", "s22", "s33", "s12", "s13", "s23")
 
 
 def _fptr(a):

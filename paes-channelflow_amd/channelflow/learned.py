@@ -1,0 +1,279 @@
+"""Training pipeline of the learned SGS closure (no counterpart in the reference).
+
+The samples come from this code's own finer-grid run:
+
+1. A fine FP32 run of the channel, from the seeded log-law initial state.
+2. Every few steps after spin-up, the device box filter (cf_filter_snapshot)
+   averages the state over fx x fy x fz fine cells.
+3. The coarse fields give the 11 closure inputs, with the stencils the LES
+   itself uses at a cell centre (the same as k_rows_mlp):
+   - spectral ux uy vx vy of the band-limited (Nyquist-free) coarse field;
+   - uz, vz = (u(K+1) - u(K-1)) / (2 dz) with zero ghost levels;
+   - wx, wy = the mean of the two coarse faces;
+   - wz = (w(K) - w(K-1)) / dz;
+   - the coarse filter width (dx dy dz)^(1/3);
+   - the wall distance.
+4. The targets are the complete subfilter stresses: the resolved part
+   <u_i u_j> - <u_i><u_j> of the cell-centred fine velocities, plus the box
+   mean of the fine run's own TKE-closure stresses.
+5. The network is trained on the device by the library's BF16 MFMA forward,
+   backward and Adam kernels (SgsNet.train).
+
+``evaluate`` measures a trained network a priori on held-out samples.  It
+compares the deviatoric stress error against three baselines:
+- the TKE closure's -2 K S_ij with K = 0.1 dsl sqrt(e) (complen.f90:88-89);
+- that closure with its best least-squares coefficient;
+- no model.
+Here e is the resolved subfilter energy plus the filtered fine-grid e.
+
+Run ``python -m channelflow.learned --help`` (from paes-channelflow_amd/).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (CF_PREC_FP32, ChannelFlow, RunParams, SgsNet, z0_of_retau)
+
+FEATURE_NAMES = ("ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz", "delta", "wall")
+STRESS_NAMES = ("t11", "t22", "t33", "t12", "t13", "t23")
+
+
+def _deal(a):
+    """Remove the x and y Nyquist modes of each plane (filter_fftw.f90:316-329)."""
+    F = np.fft.rfft2(a)
+    ny, nx = a.shape[-2:]
+    F[..., :, nx // 2] = 0.0
+    F[..., ny // 2, :] = 0.0
+    return np.fft.irfft2(F, s=(ny, nx))
+
+
+def _ddx(a, length):
+    F = np.fft.rfft2(a)
+    ny, nx = a.shape[-2:]
+    k = 2 * math.pi / length * np.arange(nx // 2 + 1)
+    k[nx // 2] = 0.0
+    return np.fft.irfft2(F * (1j * k), s=(ny, nx))
+
+
+def _ddy(a, length):
+    F = np.fft.rfft2(a)
+    ny, nx = a.shape[-2:]
+    k = 2 * math.pi / length * np.fft.fftfreq(ny, 1.0 / ny)
+    k[ny // 2] = 0.0
+    return np.fft.irfft2(F * (1j * k)[:, None], s=(ny, nx))
+
+
+def coarse_samples(fd, xl, yl, zl):
+    """Features (N, 11), targets (N, 6) and the subfilter SGS energy (N,) from
+    one filtered snapshot ``fd`` (ChannelFlow.filter_snapshot output)."""
+    u = _deal(fd["u"])
+    v = _deal(fd["v"])
+    wf = _deal(fd["wf"])                      # faces 1..ncz, top = 0
+    ncz, ncy, ncx = u.shape
+    dz = zl / ncz
+    zero = np.zeros((1, ncy, ncx))
+    ue = np.concatenate([zero, u, zero])      # levels 0..ncz+1
+    ve = np.concatenate([zero, v, zero])
+    we = np.concatenate([zero, wf])           # faces 0..ncz
+    wxf = _ddx(we, xl)
+    wyf = _ddy(we, yl)
+    zc = (np.arange(1, ncz + 1) - 0.5) * dz
+    wall = np.broadcast_to(np.minimum(zc, zl - zc)[:, None, None], u.shape)
+    delta = (xl / ncx * yl / ncy * dz) ** (1.0 / 3.0)
+    feats = np.stack([
+        _ddx(u, xl), _ddy(u, yl), (ue[2:] - ue[:-2]) / (2 * dz),
+        _ddx(v, xl), _ddy(v, yl), (ve[2:] - ve[:-2]) / (2 * dz),
+        0.5 * (wxf[:-1] + wxf[1:]), 0.5 * (wyf[:-1] + wyf[1:]), (we[1:] - we[:-1]) / dz,
+        np.full(u.shape, delta), wall], axis=-1).reshape(-1, 11)
+    ub, vb, wb = fd["u"], fd["v"], fd["wc"]
+    tau = np.stack([fd["uu"] - ub * ub, fd["vv"] - vb * vb, fd["ww"] - wb * wb,
+                    fd["uv"] - ub * vb, fd["uw"] - ub * wb, fd["vw"] - vb * wb], axis=-1)
+    if "s11" in fd:                              # plus the fine grid's own SGS stresses
+        tau = tau + np.stack([fd[k] for k in ("s11", "s22", "s33", "s12", "s13", "s23")], axis=-1)
+    tau = tau.reshape(-1, 6)
+    e_sgs = 0.5 * (tau[:, 0] + tau[:, 1] + tau[:, 2]) + fd["e"].reshape(-1)
+    return feats.astype(np.float32), tau.astype(np.float32), e_sgs.astype(np.float32)
+
+
+def deviatoric(t):
+    d = np.array(t, dtype=np.float64, copy=True)
+    tr = (d[:, 0] + d[:, 1] + d[:, 2]) / 3.0
+    d[:, :3] -= tr[:, None]
+    return d
+
+
+def tke_stress(feats, e_sgs, coef=0.1):
+    """-2 K S_ij, K = coef * delta * sqrt(e) (complen.f90:88-89 with alk = dsl)."""
+    f = feats.astype(np.float64)
+    K = coef * f[:, 9] * np.sqrt(np.maximum(e_sgs.astype(np.float64), 0.0))
+    ux, uy, uz, vx, vy, vz, wx, wy, wz = (f[:, i] for i in range(9))
+    return np.stack([-2 * K * ux, -2 * K * vy, -2 * K * wz, -K * (uy + vx), -K * (uz + wx),
+                     -K * (vz + wy)], axis=-1)
+
+
+def _dev_err(pred, true):
+    """Mean squared deviatoric stress error, off-diagonals counted twice
+    (the Frobenius norm of the symmetric tensor)."""
+    d = deviatoric(pred) - deviatoric(true)
+    w = np.array([1, 1, 1, 2, 2, 2], float)
+    return float(np.mean((d * d) @ w))
+
+
+def evaluate(pred, feats, tau, e_sgs):
+    """A-priori comparison on held-out samples; see the module docstring."""
+    tke = tke_stress(feats, e_sgs)
+    dt, dk = deviatoric(tau), deviatoric(tke)
+    w = np.array([1, 1, 1, 2, 2, 2], float)
+    c = float(((dt * dk) @ w).sum() / max(((dk * dk) @ w).sum(), 1e-300))
+    S = np.stack([feats[:, 0], feats[:, 4], feats[:, 8], 0.5 * (feats[:, 1] + feats[:, 3]),
+                  0.5 * (feats[:, 2] + feats[:, 6]), 0.5 * (feats[:, 5] + feats[:, 7])], -1).astype(np.float64)
+
+    def diss(t):
+        return -(deviatoric(t) * S) @ w
+
+    eps_true = diss(tau)
+    out = {
+        "n": int(tau.shape[0]),
+        "mse_dev": {"mlp": _dev_err(pred, tau), "tke": _dev_err(tke, tau),
+                    "tke_best_coef": _dev_err(c * tke, tau), "zero": _dev_err(0 * tau, tau)},
+        "tke_best_coef_scale": c,
+        "dissipation_corr": {"mlp": float(np.corrcoef(diss(pred), eps_true)[0, 1]),
+                             "tke": float(np.corrcoef(diss(tke), eps_true)[0, 1])},
+        "mean_dissipation": {"true": float(eps_true.mean()), "mlp": float(diss(pred).mean()),
+                             "tke": float(diss(tke).mean())},
+    }
+    return out
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def run(args):
+    os.makedirs(args.out_dir, exist_ok=True)
+    nx, nz = args.nx, args.nz
+    z0 = z0_of_retau(args.re_tau)
+    dz = 2.0 / nz
+    dt = args.cfl / (3.2 * (3 * nx / 2) / (2 * math.pi) + 0.5 / dz)
+    g = ChannelFlow(RunParams(nx, nx, nz, z0=z0, dt=dt, precision=CF_PREC_FP32))
+    g.seeded_init(args.seed, 0.1)
+    t0 = time.time()
+    done = 0
+    while done < args.spinup:
+        k = min(250, args.spinup - done)
+        g.step(k)
+        g.sync()
+        done += k
+        _log(f"[learned] spin-up {done}/{args.spinup} steps, {time.time() - t0:.0f} s, "
+             f"utau {g.scalars()['utau']:.4f}")
+    filters = [tuple(int(x) for x in f.split(",")) for f in args.filters]
+    per = []
+    nheld = 0
+    held = None
+    chunks_f, chunks_t, chunks_e = [], [], []
+    for s in range(args.snaps):
+        g.step(args.every)
+        g.sync()
+        for (fx, fy, fz) in filters:
+            fd = g.filter_snapshot(fx, fy, fz)
+            f, t, e = coarse_samples(fd, 2 * math.pi, math.pi, 2.0)
+            if s == args.snaps - 1:
+                held = (f, t, e) if held is None else tuple(np.concatenate([a, b]) for a, b in zip(held, (f, t, e)))
+                nheld += f.shape[0]
+            else:
+                chunks_f.append(f); chunks_t.append(t); chunks_e.append(e)
+        per.append(time.time() - t0)
+        _log(f"[learned] snapshot {s + 1}/{args.snaps}, {time.time() - t0:.0f} s")
+    state_stats = {"utau": g.scalars()["utau"], "steps": args.spinup + args.snaps * args.every, "dt": dt}
+    g.close()
+    ntrain = sum(x.shape[0] for x in chunks_f)
+    # standardisation from a subsample of the training set
+    sub = np.concatenate([x[:: max(1, x.shape[0] // 20000)] for x in chunks_f])
+    subt = np.concatenate([x[:: max(1, x.shape[0] // 20000)] for x in chunks_t])
+    in_mu = sub.mean(0); in_sd = sub.std(0)
+    in_sd[in_sd < 1e-12 * np.maximum(1.0, np.abs(in_mu))] = 1.0
+    out_mu = subt.mean(0); out_sd = subt.std(0)
+    net = SgsNet(0)
+    net.set_norm(in_mu, in_sd, out_mu, out_sd)
+    rng = np.random.default_rng(args.seed)
+    net.set_params(_init_params(rng))
+    net.reserve(ntrain)
+    off = 0
+    for f, t in zip(chunks_f, chunks_t):
+        net.put_samples(off, f, t)
+        off += f.shape[0]
+    del chunks_f, chunks_t, chunks_e
+    _log(f"[learned] {ntrain} training samples on the device, {nheld} held out")
+    hist = []
+    step = 0
+    for (nsteps, lr) in args.schedule:
+        for k in range(0, nsteps, 500):
+            m = min(500, nsteps - k)
+            losses = net.train(m, args.batch, lr, seed=args.seed + step)
+            step += m
+            hist.append((step, float(losses[-50:].mean())))
+            _log(f"[learned] train step {step}, lr {lr:g}, batch loss {hist[-1][1]:.4f}, "
+                 f"{time.time() - t0:.0f} s")
+    f, t, e = held
+    pred = net.forward(f)
+    ev = evaluate(pred, f, t, e)
+    _log("[learned] held-out a-priori:", json.dumps(ev))
+    wpath = os.path.join(args.out_dir, "sgs_mlp.cfmlp")
+    net.save(wpath)
+    meta = {
+        "fine_grid": [nx, nx, nz], "re_tau": args.re_tau, "z0": z0, "dt": dt, "cfl": args.cfl,
+        "seed": args.seed, "spinup_steps": args.spinup, "snapshots": args.snaps, "every": args.every,
+        "filters": filters, "train_samples": ntrain, "held_out_samples": nheld, "batch": args.batch,
+        "schedule": args.schedule, "loss_history": hist, "held_out": ev, "run": state_stats,
+        "wall_s": time.time() - t0,
+    }
+    with open(os.path.join(args.out_dir, "sgs_mlp.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+    # a small held-out fixture for tests
+    idx = rng.choice(f.shape[0], size=min(args.fixture, f.shape[0]), replace=False)
+    np.savez(os.path.join(args.out_dir, "sgs_heldout.npz"), feats=f[idx], tau=t[idx], e_sgs=e[idx])
+    net.close()
+    _log(f"[learned] wrote {wpath}")
+
+
+def _init_params(rng):
+    """He-uniform initial weights (smaller on the output layer), zero biases."""
+    layers = ((64, 11), (64, 64), (64, 64), (6, 64))
+    out = []
+    for (m, n) in layers:
+        lim = math.sqrt(6.0 / n) * (0.5 if m == 6 else 1.0)
+        out.append(rng.uniform(-lim, lim, (m, n)).ravel())
+        out.append(np.zeros(m))
+    return np.concatenate(out).astype(np.float32)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--re-tau", type=float, default=395.0)
+    ap.add_argument("--cfl", type=float, default=0.3)
+    ap.add_argument("--seed", type=int, default=10972)
+    ap.add_argument("--spinup", type=int, default=2000)
+    ap.add_argument("--snaps", type=int, default=33, help="snapshots; the last one is held out")
+    ap.add_argument("--every", type=int, default=25)
+    ap.add_argument("--filters", nargs="+", default=["4,4,2", "4,4,4"])
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--schedule", type=lambda s: [(int(a), float(b)) for a, b in
+                                                  (p.split(":") for p in s.split(","))],
+                    default=[(4000, 1e-3), (3000, 3e-4), (3000, 1e-4)],
+                    help="steps:lr,steps:lr,...")
+    ap.add_argument("--fixture", type=int, default=16384)
+    ap.add_argument("--out-dir", required=True)
+    run(ap.parse_args(argv))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1412,6 +1412,21 @@ __global__ void k_reduce_part(const double* __restrict__ part, size_t pstride, i
   }
 }
 
+// i kx (dir 0) or i ky (dir 1) times a band spectrum, slots 1..nz.
+template <typename T>
+__global__ void __launch_bounds__(256) k_band_deriv(const cx<T>* __restrict__ in, cx<T>* out,
+                                                    const T* __restrict__ kxp, const T* __restrict__ kyp,
+                                                    int mx, int ny, int nz, int dir) {
+  const size_t plane = (size_t)mx * ny;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= plane * nz) return;
+  const size_t o = plane + i;                   // slot 1 onwards
+  const int k = (int)(i % mx), j = (int)((i / mx) % ny);
+  const T kk = dir == 0 ? kxp[k] : kyp[j];
+  const cx<T> v = in[o];
+  out[o] = cx<T>{-kk * v.y, kk * v.x};
+}
+
 template <typename A, typename B>
 __global__ void k_cast(const A* __restrict__ in, B* out, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -100,12 +100,17 @@ __device__ __forceinline__ bf16x8 shfl_xor32(bf16x8 v) {
 }
 
 // Hidden layer with K-steps S: D[mt] = b + W B for both 32-unit output tiles.
+// Keeps the scheduler from hoisting one layer's weight loads into another's
+// (each hoisted A operand costs 4 VGPRs in the row kernel).
+__device__ __forceinline__ void layer_fence() { asm volatile("" ::: "memory"); }
+
 template <int S>
 __device__ __forceinline__ void hidden(const Net* net, int ft, int fb, const bf16x8 (&B)[4],
                                        f32x16 (&D)[2], int lane) {
   const int half = lane >> 5;
 #pragma unroll
   for (int mt = 0; mt < 2; ++mt) {
+    layer_fence();
     D[mt] = load16(net->fb + fb + (mt * 2 + half) * 16);
 #pragma unroll
     for (int s = 0; s < S; ++s) D[mt] = mfma(net->fw[(ft + mt * S + s) * 64 + lane], B[s], D[mt]);
@@ -134,6 +139,7 @@ __device__ __forceinline__ f32x16 forward_tile(const Net* net, bf16x8 xin, int l
       B[2 * mt + 1] = pack8_gelu(D[mt], 8);
     }
   }
+  layer_fence();
   f32x16 Y = load16(net->fb + FB4 + half * 16);
 #pragma unroll
   for (int s = 0; s < 4; ++s) Y = mfma(net->fw[(FT4 + s) * 64 + lane], B[s], Y);
@@ -555,21 +561,36 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const Net* __restrict__ gnet
 // ---------------------------------------------------------------------------
 // Training data: box filter of a fine-grid state onto a coarse grid
 // ---------------------------------------------------------------------------
-// Input: physical u, v, e at levels 1..nz and w at faces 1..nz (w(nz) = 0),
-// each [nz][ny][nx].  The coarse cell (I, J, K) covers fine points
-// fx*I .. fx*I+fx-1, fy*J .., and levels fz*K+1 .. fz*K+fz.  Output [11][ncz][ncy][ncx]:
-//   0 u, 1 v, 2 w at cell centres, 3 w at the coarse face K+1 (fine face
-//   fz*(K+1), trapezoid over fine faces fz*(K+1) -/+ fz/2, zero at the top
-//   wall), 4 e at cell centres, 5..10 uu vv ww uv uw vw,
-// all box means.  Cell-centred w is (w(k-1)+w(k))/2 with w(0) = 0; cell-centred
-// e is (e(k-1)+e(k))/2 with e(0) = e(1) (lowerbc.f90:13).
+// Input: physical u, v, e at levels 1..nz, w at faces 1..nz (w(nz) = 0), and
+// the spectral derivatives ux uy vx vy (levels) and wx wy (faces), each
+// [nz][ny][nx].  The coarse cell (I, J, K) covers fine points fx*I .. fx*I+fx-1,
+// fy*J .., and levels fz*K+1 .. fz*K+fz.  Output [NFILT][ncz][ncy][ncx], all
+// box means:
+//   0 u, 1 v, 2 w (cell centre), 3 w at the coarse face K+1 (fine face
+//   fz*(K+1), trapezoid over the fine faces fz*(K+1) -/+ fz/2, zero at the
+//   top wall), 4 e (cell centre), 5..10 uu vv ww uv uw vw;
+//   11..16 the fine grid's own SGS stresses t11 t22 t33 t12 t13 t23 at cell
+//   centres.  These follow the TKE closure (complen.f90:88-89,
+//   compsgs1.f90:144-153), with K = min(0.1 alk sqrt(e), xkmax) at faces:
+//   -(K(k-1)+K(k)) ux, ..., and the face stresses -K (uz + wx) averaged to
+//   the centre (the wall faces take the adjacent interior face's value).
+// Cell-centred w is (w(k-1)+w(k))/2 with w(0) = 0; cell-centred e is
+// (e(k-1)+e(k))/2 with e(0) = e(1) (lowerbc.f90:13).
+constexpr int NFILT = 17;
 struct FilterArgs {
   const double* u;
   const double* v;
   const double* w;
   const double* e;
+  const double* ux;
+  const double* uy;
+  const double* vx;
+  const double* vy;
+  const double* wx;
+  const double* wy;
   double* out;
   int nx, ny, nz, fx, fy, fz;
+  double dz, alk, xkmax;
 };
 
 __global__ void __launch_bounds__(256) k_box_filter(FilterArgs a) {
@@ -579,42 +600,66 @@ __global__ void __launch_bounds__(256) k_box_filter(FilterArgs a) {
   if (c >= nc) return;
   const int I = (int)(c % ncx), J = (int)((c / ncx) % ncy), K = (int)(c / ((int64_t)ncx * ncy));
   const size_t plane = (size_t)a.nx * a.ny;
-  double s[11] = {};
+  const double dzm = 1.0 / a.dz;
+  auto at = [&](const double* f, int lev, size_t p) { return f[(size_t)(lev - 1) * plane + p]; };
+  auto kf = [&](int face, size_t p) {          // eddy diffusivity at face 0..nz
+    const double e = at(a.e, face < 1 ? 1 : face, p);
+    const double k = 0.1 * a.alk * sqrt(e);
+    return k < a.xkmax ? k : a.xkmax;
+  };
+  auto t13f = [&](int face, size_t p, double& t13, double& t23) {   // face 1..nz-1
+    const double kk = kf(face, p);
+    const double uz = (at(a.u, face + 1, p) - at(a.u, face, p)) * dzm;
+    const double vz = (at(a.v, face + 1, p) - at(a.v, face, p)) * dzm;
+    t13 = -kk * (uz + at(a.wx, face, p));
+    t23 = -kk * (vz + at(a.wy, face, p));
+  };
+  double s[NFILT] = {};
   for (int kk = 0; kk < a.fz; ++kk) {
     const int k = a.fz * K + kk + 1;                    // fine level 1..nz
     for (int jj = 0; jj < a.fy; ++jj) {
       const size_t rowo = (size_t)(a.fy * J + jj) * a.nx;
       for (int ii = 0; ii < a.fx; ++ii) {
         const size_t p = rowo + a.fx * I + ii;
-        const double uu = a.u[(size_t)(k - 1) * plane + p];
-        const double vv = a.v[(size_t)(k - 1) * plane + p];
-        const double wlo = k >= 2 ? a.w[(size_t)(k - 2) * plane + p] : 0.0;
-        const double wc = 0.5 * (wlo + a.w[(size_t)(k - 1) * plane + p]);
-        const double elo = a.e[(size_t)(k >= 2 ? k - 2 : 0) * plane + p];
-        const double ec = 0.5 * (elo + a.e[(size_t)(k - 1) * plane + p]);
+        const double uu = at(a.u, k, p);
+        const double vv = at(a.v, k, p);
+        const double wlo = k >= 2 ? at(a.w, k - 1, p) : 0.0;
+        const double whi = at(a.w, k, p);
+        const double wc = 0.5 * (wlo + whi);
+        const double ec = 0.5 * (at(a.e, k >= 2 ? k - 1 : 1, p) + at(a.e, k, p));
         s[0] += uu; s[1] += vv; s[2] += wc; s[4] += ec;
         s[5] += uu * uu; s[6] += vv * vv; s[7] += wc * wc;
         s[8] += uu * vv; s[9] += uu * wc; s[10] += vv * wc;
+        const double xs = kf(k - 1, p) + kf(k, p);
+        s[11] += -xs * at(a.ux, k, p);
+        s[12] += -xs * at(a.vy, k, p);
+        s[13] += -xs * (whi - wlo) * dzm;
+        s[14] += -xs * 0.5 * (at(a.uy, k, p) + at(a.vx, k, p));
+        double a13, a23, b13, b23;
+        t13f(k - 1 < 1 ? 1 : (k - 1 > a.nz - 1 ? a.nz - 1 : k - 1), p, a13, a23);
+        t13f(k > a.nz - 1 ? a.nz - 1 : k, p, b13, b23);
+        s[15] += 0.5 * (a13 + b13);
+        s[16] += 0.5 * (a23 + b23);
       }
     }
   }
   // face filter: fine faces kf-h .. kf+h, end weights halved (h = fz/2)
-  const int kf = a.fz * (K + 1);
+  const int kfc = a.fz * (K + 1);
   double wf = 0.0, wsum = 0.0;
   if (K + 1 < ncz) {
     const int h = a.fz / 2;
     for (int d = -h; d <= h; ++d) {
       const double wt = (a.fz > 1 && (d == -h || d == h)) ? 0.5 : 1.0;
-      const int k = kf + d;                                 // fine face, 1..nz-1 here
+      const int k = kfc + d;                                // fine face, 1..nz-1 here
       for (int jj = 0; jj < a.fy; ++jj)
         for (int ii = 0; ii < a.fx; ++ii)
-          wf += wt * a.w[(size_t)(k - 1) * plane + (size_t)(a.fy * J + jj) * a.nx + a.fx * I + ii];
+          wf += wt * at(a.w, k, (size_t)(a.fy * J + jj) * a.nx + a.fx * I + ii);
       wsum += wt;
     }
     wf /= wsum * a.fx * a.fy;
   }
   const double inv = 1.0 / ((double)a.fx * a.fy * a.fz);
-  for (int q = 0; q < 11; ++q) a.out[(size_t)q * nc + c] = q == 3 ? wf : s[q] * inv;
+  for (int q = 0; q < NFILT; ++q) a.out[(size_t)q * nc + c] = q == 3 ? wf : s[q] * inv;
 }
 
 }  // namespace mlp

@@ -135,6 +135,8 @@ struct Engine {
   virtual int get_field(cf_ctx* ctx, int which, double* out) = 0;
   // the same field, physical, left in ctx->staging (enqueued on the stream)
   virtual int physical(cf_ctx* ctx, int which) = 0;
+  // d/dx (dir 0) or d/dy (dir 1) of u, v or w (fld 0..2), physical, in ctx->staging
+  virtual int physical_deriv(cf_ctx* ctx, int fld, int dir) = 0;
 };
 
 template <typename T, int NX>
@@ -425,6 +427,15 @@ struct EngineT : Engine {
     HIPCHK(hipMemcpyAsync(out, ctx->staging, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CF_OK;
+  }
+
+  int physical_deriv(cf_ctx* ctx, int fld, int dir) override {
+    const size_t n = (size_t)ctx->nz * NY * MX;
+    void* src = fld == 0 ? ctx->U : (fld == 1 ? ctx->V : ctx->W);
+    hipLaunchKernelGGL((cf::k_band_deriv<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       cptr(src), cptr(ctx->band_tmp), rptr(ctx->kxp), rptr(ctx->kyp), MX, NY, ctx->nz, dir);
+    HIPCHK(hipGetLastError());
+    return from_band<T>(ctx, cptr(ctx->band_tmp), cptr(ctx->xs_tmp), cptr(ctx->twNX));
   }
 
   int physical(cf_ctx* ctx, int which) override {
@@ -1249,26 +1260,38 @@ int cf_filter_snapshot(cf_ctx* ctx, int32_t fx, int32_t fy, int32_t fz, double* 
   HIPCHK(hipSetDevice(ctx->p.device));
   const size_t n = (size_t)ctx->nz * ctx->ny * ctx->nx;
   const size_t nc = n / ((size_t)fx * fy * fz);
+  constexpr int NF = mlpns::NFILT;
   if (!ctx->filt_in) {
-    int rc = dalloc(ctx, &ctx->filt_in, 4 * n);
+    int rc = dalloc(ctx, &ctx->filt_in, 10 * n);
     if (rc != CF_OK) return rc;
   }
-  if (ctx->filt_out_n < 11 * nc) {
-    int rc = dalloc(ctx, &ctx->filt_out, 11 * nc);
+  if (ctx->filt_out_n < NF * nc) {
+    int rc = dalloc(ctx, &ctx->filt_out, NF * nc);
     if (rc != CF_OK) return rc;
-    ctx->filt_out_n = 11 * nc;
+    ctx->filt_out_n = NF * nc;
   }
-  for (int f = 0; f < 4; ++f) {
-    int rc = ctx->eng->physical(ctx, f);
+  // u, v, w, e, then ux, uy, vx, vy, wx, wy
+  for (int f = 0; f < 10; ++f) {
+    int rc = f < 4 ? ctx->eng->physical(ctx, f) : ctx->eng->physical_deriv(ctx, (f - 4) / 2, (f - 4) % 2);
     if (rc != CF_OK) return rc;
     HIPCHK(hipMemcpyAsync(ctx->filt_in + f * n, ctx->staging, n * sizeof(double),
                           hipMemcpyDeviceToDevice, ctx->stream));
   }
-  mlpns::FilterArgs fa{ctx->filt_in, ctx->filt_in + n, ctx->filt_in + 2 * n, ctx->filt_in + 3 * n,
-                       ctx->filt_out, ctx->nx, ctx->ny, ctx->nz, fx, fy, fz};
+  const double dz = ctx->p.zl / ctx->nz;
+  const double dsl = std::pow(ctx->p.xl / ctx->nx * ctx->p.yl / ctx->ny * dz, 1.0 / 3.0);
+  mlpns::FilterArgs fa{};
+  const double* fi = ctx->filt_in;
+  fa.u = fi; fa.v = fi + n; fa.w = fi + 2 * n; fa.e = fi + 3 * n;
+  fa.ux = fi + 4 * n; fa.uy = fi + 5 * n; fa.vx = fi + 6 * n; fa.vy = fi + 7 * n;
+  fa.wx = fi + 8 * n; fa.wy = fi + 9 * n;
+  fa.out = ctx->filt_out;
+  fa.nx = ctx->nx; fa.ny = ctx->ny; fa.nz = ctx->nz; fa.fx = fx; fa.fy = fy; fa.fz = fz;
+  fa.dz = dz;
+  fa.alk = ctx->p.alk > 0 ? ctx->p.alk : dsl;
+  fa.xkmax = dz * dz / ctx->dt / 5.0;                       // les.F90:216
   hipLaunchKernelGGL(mlpns::k_box_filter, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream, fa);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, ctx->filt_out, 11 * nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(out, ctx->filt_out, NF * nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return CF_OK;
 }

@@ -12,7 +12,8 @@
 !
 ! One optional namelist group is added:
 !   &GPU precision=0|1 (0: FP64, 1: FP32 fields with FP64 pressure),
-!        device, alk (<= 0: dsl), seed, amp /
+!        device, alk (<= 0: dsl), seed, amp,
+!        closure=0|1 (1: learned SGS closure with TKE fallback), weights='file' /
 ! seed/amp select the seeded initial condition used when restart=.false.
 ! (the reference's initvel draws from the compiler's random_number).
 ! Restart files may hold REAL(4) records (the shipped reference) or REAL(8)
@@ -79,6 +80,30 @@ module channelflow_c
        import :: c_ptr
        type(c_ptr), value :: ctx
      end function cf_last_error
+     ! learned SGS closure (no counterpart in the reference)
+     integer(c_int) function cf_mlp_create(device, m) bind(C, name="cf_mlp_create")
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: device
+       type(c_ptr), intent(out) :: m
+     end function cf_mlp_create
+     integer(c_int) function cf_mlp_load(m, path) bind(C, name="cf_mlp_load")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: m
+       character(kind=c_char), intent(in) :: path(*)
+     end function cf_mlp_load
+     integer(c_int) function cf_mlp_destroy(m) bind(C, name="cf_mlp_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: m
+     end function cf_mlp_destroy
+     integer(c_int) function cf_set_closure(ctx, m) bind(C, name="cf_set_closure")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, m
+     end function cf_set_closure
+     integer(c_int) function cf_get_closure_counts(ctx, counts) bind(C, name="cf_get_closure_counts")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: counts(2)
+     end function cf_get_closure_counts
   end interface
 
 contains
@@ -120,18 +145,20 @@ program les_gpu
   character(100) :: restart_file, case
   integer :: timax, savfrq, imodel, isurface, nnx, nny, nnz
   real(8) :: meanfrq, cfl, dt, cb, skc, xl, yl, zl, z0
-  integer :: precision, device
+  integer :: precision, device, closure
   integer(8) :: seed
   real(8) :: alk, amp
+  character(256) :: weights
   namelist /RESTARTING/ restart, restart_file
   namelist /TIMING/ case, timax, savfrq, meanfrq, isbgr, cfl, dt, &
        cb, skc, spectout, spectav, imodel, isurface
   namelist /GRIDSIZE/ nnx, nny, nnz
   namelist /PARAMS/ xl, yl, zl, z0
-  namelist /GPU/ precision, device, alk, seed, amp
+  namelist /GPU/ precision, device, alk, seed, amp, closure, weights
 
   type(cf_params) :: p
-  type(c_ptr) :: ctx
+  type(c_ptr) :: ctx, net
+  integer(c_int64_t) :: counts(2)
   real(8), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:), e(:,:,:), pr(:,:,:), meanbuf(:,:)
   real(8) :: time, time_start, utau, divgls, fcor, ugtop, ugbot, vgtop, vgbot, s4(4)
   real(8) :: seconds, wallclock0, wallclock1, zm, zt
@@ -146,6 +173,7 @@ program les_gpu
   nnx = 32; nny = 32; nnz = 32; xl = 5000.0d0; yl = 5000.0d0; zl = 2000.0d0; z0 = 0.16d0
   cfl = 0.3d0; dt = 0.01d0; cb = 0; skc = 0; imodel = 3; isurface = 1
   precision = CF_PREC_FP64; device = 0; alk = 0.0d0; seed = 10972_8; amp = 0.1d0
+  closure = 0; weights = "sgs_mlp.cfmlp"; net = c_null_ptr
   divgls = 0.0d0; fcor = 0.0001d0; ugtop = 15.0d0; ugbot = 15.0d0; vgtop = 0.0d0; vgbot = 0.0d0
 
   write(*,*)
@@ -192,6 +220,15 @@ program les_gpu
   p%imodel = imodel; p%isurface = isurface; p%precision = precision; p%device = device
   p%alk = alk; p%ugal = 0.0d0; p%vgal = 0.0d0
   call check(cf_create(p, ctx), c_null_ptr, "cf_create")
+  if (closure == 1) then
+     call check(cf_mlp_create(int(device, c_int32_t), net), c_null_ptr, "cf_mlp_create")
+     if (cf_mlp_load(net, trim(weights)//c_null_char) /= 0) then
+        write(*,*) "Stopping; cannot load the closure network ", trim(weights)
+        stop 1
+     end if
+     call check(cf_set_closure(ctx, net), ctx, "cf_set_closure")
+     write(*,*) "Learned SGS closure from ", trim(weights), " (TKE fallback)"
+  end if
 
   allocate(u(nnx,nny,nnz), v(nnx,nny,nnz), w(nnx,nny,nnz), e(nnx,nny,nnz), pr(nnx,nny,nnz))
   allocate(meanbuf(19,nnz))
@@ -289,6 +326,13 @@ program les_gpu
      it = it + 1
   end do
 
+  if (closure == 1) then
+     call check(cf_get_closure_counts(ctx, counts), ctx, "cf_get_closure_counts")
+     write(*,'(a,i0,a,i0,a)') " Learned closure used at ", counts(1), " of ", counts(2), &
+          " cell evaluations since the last output"
+     call check(cf_set_closure(ctx, c_null_ptr), ctx, "cf_set_closure")
+     ios = cf_mlp_destroy(net)
+  end if
   call check(cf_destroy(ctx), c_null_ptr, "cf_destroy")
   call date_and_time(values=time_array)
   wallclock1 = time_array(5)*3600.d0 + time_array(6)*60.d0 + time_array(7) + 0.001d0*time_array(8)

@@ -79,3 +79,39 @@ def test_driver_restart_from_real4_file(tmp_path):
     for k in "uvwe":
         ref = O.interior(st, pr, k)
         assert np.linalg.norm(f[k] - ref) / np.linalg.norm(ref) < 1e-10, k
+
+
+NML_LEARNED = """&RESTARTING restart=.false., restart_file='start.field' /
+&TIMING case='chan', timax=1, savfrq=1, meanfrq=1, isbgr=.true., dt={dt}, imodel=3, isurface=1 /
+&GRIDSIZE nnx=128, nny=128, nnz=16 /
+&PARAMS xl=6.283185307179586, yl=3.141592653589793, zl=2.0, z0=3.4262e-4 /
+&GPU precision=1, seed=10972, amp=0.1, closure=1, weights='{weights}' /
+"""
+
+
+def test_driver_learned_closure_matches_the_library_run(tmp_path):
+    """closure=1 in the GPU namelist group loads the shipped network and runs
+    the FP32 path with it; the .field output equals the same run made through
+    the C-ABI from Python."""
+    import channelflow as C
+    from channelflow.fieldio import read_field
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built")
+    weights = os.path.join(ROOT, "paes-channelflow_amd", "weights", "sgs_mlp.cfmlp")
+    dt = 1.0 / 64
+    (tmp_path / "nlist.input").write_text(NML_LEARNED.format(dt=dt, weights=weights))
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Learned SGS closure" in r.stdout and "Learned closure used at" in r.stdout
+    _, f = read_field(tmp_path / "chan1.field")
+    g = C.ChannelFlow(C.RunParams(128, 128, 16, z0=3.4262e-4, dt=dt, precision=C.CF_PREC_FP32))
+    g.seeded_init(10972, 0.1)
+    net = C.SgsNet(0, weights)
+    g.set_closure(net)
+    g.step(64)
+    s = g.get_state()
+    acc, tot = g.closure_counts()
+    g.close(); net.close()
+    assert 0 < acc < tot
+    for k in "uvwe":
+        assert np.array_equal(f[k], s[k]), k

@@ -211,3 +211,67 @@ def test_network_file_round_trip(tmp_path):
         assert np.array_equal(a, b)
     assert np.array_equal(net2.forward(feats), net.forward(feats))
     net.close(); net2.close()
+
+
+def _box(a, fx, fy, fz):
+    nz, ny, nx = a.shape
+    return a.reshape(nz // fz, fz, ny // fy, fy, nx // fx, fx).mean(axis=(1, 3, 5))
+
+
+@pytest.mark.parametrize("f", [(2, 2, 2), (4, 4, 2), (4, 4, 4)])
+def test_box_filter_matches_numpy(f):
+    import channelflow as C
+    fx, fy, fz = f
+    pr = O.Params(128, 128, 16, dt=0.002)
+    g = _les(pr, C.CF_PREC_FP64)
+    g.seeded_init(10972, 0.1)
+    g.step(2)
+    s = g.get_state()
+    fd = g.filter_snapshot(fx, fy, fz)
+    u, v, w, e = s["u"], s["v"], s["w"], s["e"]
+    wlo = np.concatenate([np.zeros_like(w[:1]), w[:-1]])
+    elo = np.concatenate([e[:1], e[:-1]])
+    wc, ec = 0.5 * (wlo + w), 0.5 * (elo + e)
+    ref = {"u": _box(u, fx, fy, fz), "v": _box(v, fx, fy, fz), "wc": _box(wc, fx, fy, fz),
+           "e": _box(ec, fx, fy, fz), "uu": _box(u * u, fx, fy, fz), "vv": _box(v * v, fx, fy, fz),
+           "ww": _box(wc * wc, fx, fy, fz), "uv": _box(u * v, fx, fy, fz), "uw": _box(u * wc, fx, fy, fz),
+           "vw": _box(v * wc, fx, fy, fz)}
+    # faces: fine faces fz(K+1) -/+ fz/2 with halved end weights, zero at the top
+    nz = pr.nnz
+    wts = np.ones(fz + 1) if fz == 1 else np.r_[0.5, np.ones(fz - 1), 0.5]
+    wts = wts[:1] if fz == 1 else wts
+    wf = np.zeros_like(ref["u"])
+    wxy = _box(w, fx, fy, 1)                       # x-y means at every fine face
+    for K in range(nz // fz - 1):
+        kf = fz * (K + 1)
+        ks = np.arange(kf - fz // 2, kf + fz // 2 + 1)
+        wf[K] = np.tensordot(wts, wxy[ks - 1], axes=1) / wts.sum()
+    ref["wf"] = wf
+    # the fine grid's TKE-closure stresses (compsgs1.f90:144-153) at cell centres
+    def ddx(a):
+        F = np.fft.rfft2(a); kx = 2 * np.pi / pr.xl * np.arange(F.shape[-1]); kx[-1] = 0
+        return np.fft.irfft2(F * 1j * kx, s=a.shape[-2:])
+    def ddy(a):
+        F = np.fft.rfft2(a); ky = 2 * np.pi / pr.yl * np.fft.fftfreq(a.shape[-2], 1.0 / a.shape[-2])
+        ky[a.shape[-2] // 2] = 0
+        return np.fft.irfft2(F * 1j * ky[:, None], s=a.shape[-2:])
+    dz = pr.dz
+    K = np.minimum(0.1 * pr.dsl * np.sqrt(e), dz * dz / pr.dt / 5)      # faces 1..nz
+    Klo = np.concatenate([K[:1], K[:-1]])                               # face k-1 (face 0 = face 1)
+    xs = Klo + K
+    uz = (u[1:] - u[:-1]) / dz                                          # faces 1..nz-1
+    vz = (v[1:] - v[:-1]) / dz
+    t13f = -K[:-1] * (uz + ddx(w)[:-1])
+    t23f = -K[:-1] * (vz + ddy(w)[:-1])
+    idx_lo = np.clip(np.arange(nz) - 1, 0, nz - 2)                      # face k-1 -> index
+    idx_hi = np.clip(np.arange(nz), 0, nz - 2)                          # face k
+    ref["s11"] = _box(-xs * ddx(u), fx, fy, fz)
+    ref["s22"] = _box(-xs * ddy(v), fx, fy, fz)
+    ref["s33"] = _box(-xs * (w - wlo) / dz, fx, fy, fz)
+    ref["s12"] = _box(-xs * 0.5 * (ddy(u) + ddx(v)), fx, fy, fz)
+    ref["s13"] = _box(0.5 * (t13f[idx_lo] + t13f[idx_hi]), fx, fy, fz)
+    ref["s23"] = _box(0.5 * (t23f[idx_lo] + t23f[idx_hi]), fx, fy, fz)
+    for k, r in ref.items():
+        scale = max(np.abs(r).max(), 1e-300)
+        assert np.allclose(fd[k], r, rtol=1e-10, atol=1e-12 * scale), k
+    g.close()
