@@ -98,7 +98,7 @@ def test_driver_learned_closure_matches_the_library_run(tmp_path):
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built")
     weights = os.path.join(ROOT, "paes-channelflow_amd", "weights", "sgs_mlp.cfmlp")
-    dt = 1.0 / 64
+    dt = 1.0 / 512                      # CFL ~0.3 on the 192-point 3/2 grid
     (tmp_path / "nlist.input").write_text(NML_LEARNED.format(dt=dt, weights=weights))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -108,7 +108,7 @@ def test_driver_learned_closure_matches_the_library_run(tmp_path):
     g.seeded_init(10972, 0.1)
     net = C.SgsNet(0, weights)
     g.set_closure(net)
-    g.step(64)
+    g.step(512)
     s = g.get_state()
     acc, tot = g.closure_counts()
     g.close(); net.close()
