@@ -66,6 +66,8 @@ def kernel_bytes(name, n, s, istage=2):
 # profiles/k_rows_pmc_256x256x128.md holds the runs.
 MEASURED_TRAFFIC = {
     ("k_rows", 256, 128, "fp32"): round((2 * 198014.8 + 393262.8) * 1024),
+    # profiles/k_rows_mlp_counters_256x256x128.md
+    ("k_rows_mlp", 256, 128, "fp32"): round((2 * 202209 + 393792) * 1024),
 }
 
 

@@ -632,8 +632,8 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
 // Whole wavefronts are MFMA operands, so NXB must be a multiple of 64.
 // ---------------------------------------------------------------------------
 template <typename T, int NX>
-__global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp::Net* __restrict__ gnet,
-                                                         unsigned long long* counts) {
+__device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Net* __restrict__ gnet,
+                                              unsigned long long* counts) {
   using F = RowFFT<T, NX>;
   static_assert(F::WAVE, "the learned closure needs 3/2-grid rows of whole wavefronts");
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
@@ -893,6 +893,14 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp
     atomicAdd(counts, nacc);
     atomicAdd(counts + 1, 64ull * (unsigned long long)(z1 - z0));
   }
+}
+
+// (Forcing three waves per SIMD costs spills and measured no faster: the
+// kernel is bound by the network's VALU/transcendental work, not latency.)
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp::Net* __restrict__ gnet,
+                                                         unsigned long long* counts) {
+  rows_mlp_body<T, NX>(a, gnet, counts);
 }
 
 // ---------------------------------------------------------------------------

@@ -99,21 +99,29 @@ __device__ __forceinline__ bf16x8 shfl_xor32(bf16x8 v) {
   return __builtin_bit_cast(bf16x8, u);
 }
 
-// Hidden layer with K-steps S: D[mt] = b + W B for both 32-unit output tiles.
 // Keeps the scheduler from hoisting one layer's weight loads into another's
 // (each hoisted A operand costs 4 VGPRs in the row kernel).
 __device__ __forceinline__ void layer_fence() { asm volatile("" ::: "memory"); }
 
-template <int S>
+// Hidden layer with K-steps S: tile mt of b + W B, then its GELU packed into
+// the next layer's operands Bn[2mt], Bn[2mt+1].  Tile 1's MFMAs overlap tile
+// 0's GELU, and only one accumulator tile is live.
+template <int S, bool SAVE>
 __device__ __forceinline__ void hidden(const Net* net, int ft, int fb, const bf16x8 (&B)[4],
-                                       f32x16 (&D)[2], int lane) {
+                                       bf16x8 (&Bn)[4], int lane, float* zs, int layer) {
   const int half = lane >> 5;
 #pragma unroll
   for (int mt = 0; mt < 2; ++mt) {
     layer_fence();
-    D[mt] = load16(net->fb + fb + (mt * 2 + half) * 16);
+    f32x16 D = load16(net->fb + fb + (mt * 2 + half) * 16);
 #pragma unroll
-    for (int s = 0; s < S; ++s) D[mt] = mfma(net->fw[(ft + mt * S + s) * 64 + lane], B[s], D[mt]);
+    for (int s = 0; s < S; ++s) D = mfma(net->fw[(ft + mt * S + s) * 64 + lane], B[s], D);
+    if constexpr (SAVE) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) zs[((layer * 2 + mt) * 16 + i) * 64 + lane] = D[i];
+    }
+    Bn[2 * mt] = pack8_gelu(D, 0);
+    Bn[2 * mt + 1] = pack8_gelu(D, 8);
   }
 }
 
@@ -122,27 +130,15 @@ __device__ __forceinline__ void hidden(const Net* net, int ft, int fb, const bf1
 template <bool SAVE>
 __device__ __forceinline__ f32x16 forward_tile(const Net* net, bf16x8 xin, int lane, float* zs) {
   const int half = lane >> 5;
-  bf16x8 B[4];
+  bf16x8 B[4], C[4];
   B[0] = xin;
-  f32x16 D[2];
-  hidden<1>(net, FT1, FB1, B, D, lane);
-#pragma unroll
-  for (int layer = 0; layer < 3; ++layer) {
-    if (layer > 0) hidden<4>(net, layer == 1 ? FT2 : FT3, layer == 1 ? FB2 : FB3, B, D, lane);
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      if constexpr (SAVE) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) zs[((layer * 2 + mt) * 16 + i) * 64 + lane] = D[mt][i];
-      }
-      B[2 * mt] = pack8_gelu(D[mt], 0);
-      B[2 * mt + 1] = pack8_gelu(D[mt], 8);
-    }
-  }
+  hidden<1, SAVE>(net, FT1, FB1, B, C, lane, zs, 0);
+  hidden<4, SAVE>(net, FT2, FB2, C, B, lane, zs, 1);
+  hidden<4, SAVE>(net, FT3, FB3, B, C, lane, zs, 2);
   layer_fence();
   f32x16 Y = load16(net->fb + FB4 + half * 16);
 #pragma unroll
-  for (int s = 0; s < 4; ++s) Y = mfma(net->fw[(FT4 + s) * 64 + lane], B[s], Y);
+  for (int s = 0; s < 4; ++s) Y = mfma(net->fw[(FT4 + s) * 64 + lane], C[s], Y);
   return Y;
 }
 

@@ -50,7 +50,7 @@ struct Engine;
 
 struct cf_ctx {
   cf_params p{};
-  int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16, ncu = 256;
+  int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16, zchunk_mlp = 32, ncu = 256;
   bool fp32 = false;
   hipStream_t stream = nullptr;
   std::string err;
@@ -274,15 +274,18 @@ struct EngineT : Engine {
       a.pstride = (size_t)ctx->slots * NYB;
       a.flags = &ctx->ws->flags;
       a.c = c;
-      const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
       bool learned = false;
       if constexpr ((3 * NX / 2) % 64 == 0) {
         if (ctx->closure_on) {
-          hipLaunchKernelGGL((cf::k_rows_mlp<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a,
+          // longer chunks: each chunk re-evaluates the network at two extra levels
+          a.c.zchunk = ctx->zchunk_mlp;
+          const int nchm = (nz + ctx->zchunk_mlp - 1) / ctx->zchunk_mlp;
+          hipLaunchKernelGGL((cf::k_rows_mlp<T, NX>), dim3(NYB, nchm), dim3(RS::NT), 0, s, a,
                              ctx->closure_net, ctx->closure_counts);
           learned = true;
         }
       }
+      const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
       if (!learned) hipLaunchKernelGGL((cf::k_rows<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
     }
     mark();
@@ -543,6 +546,7 @@ int cf_create(const cf_params* p, cf_ctx** out) {
   ctx->fp32 = p->precision == CF_PREC_FP32;
   ctx->dt = p->dt;
   ctx->zchunk = nz >= 64 ? 16 : 8;
+  ctx->zchunk_mlp = nz >= 64 ? 32 : ctx->zchunk;
   if (hipSetDevice(p->device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "hipSetDevice/hipStreamCreate failed";
     delete ctx;

@@ -143,22 +143,24 @@ def _closure_net(kind):
     return flat, norm
 
 
-@pytest.mark.parametrize("kind", ["reject", "mixed"])
-def test_learned_stage_matches_oracle(kind):
+@pytest.mark.parametrize("kind,nz,stages", [("reject", 8, (1, 2, 3)), ("mixed", 8, (1, 2, 3)),
+                                            ("mixed", 64, (1,))])
+def test_learned_stage_matches_oracle(kind, nz, stages):
+    """nz = 8: one z-chunk of 8 levels; nz = 64: two chunks of 32 levels."""
     import channelflow as C
-    pr0 = O.Params(128, 128, 8, dt=0.002)
+    pr0 = O.Params(128, 128, nz, dt=0.002)
     u, v, w, e = O.initial_fields(pr0)
     flat, norm = _closure_net(kind)
     onet = M.Network(flat, *[np.asarray(a, float) for a in norm], bf16_mode=True)
     cl = M.LearnedClosure(onet)
-    pr = O.Params(128, 128, 8, dt=0.002, closure=cl)
+    pr = O.Params(128, 128, nz, dt=0.002, closure=cl)
     st = O.set_state(pr, u, v, w, e)
     g = _les(pr, C.CF_PREC_FP64)
     g.set_state(u, v, w, e)
     net = make_net(flat, norm)
     g.set_closure(net)
     g.closure_counts()
-    for istage in (1, 2, 3):
+    for istage in stages:
         O.stage(st, pr, istage)
         g.stage(istage)
         acc, tot = g.closure_counts()
