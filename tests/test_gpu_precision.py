@@ -1,0 +1,122 @@
+"""Precision modes against the FP64 path, at the full 256^2 x 128 workload
+(Re_tau 395, seeded log-law + noise, dt at CFL 0.3), and the long 32^3 run.
+
+* The FP32 path (FP32 fields, transforms, products and SGS; FP64 pressure,
+  projection and update) after 200 steps, against the FP64 run from the same
+  state.  The error is printed per output and asserted within:
+  - u, v, w, e, p: 1e-3 relative L2;
+  - the mean profile uxym: 1e-4 relative;
+  - the resolved TKE englez: 1e-3 relative;
+  - the calc_spectra x-y spectra: 1e-3 relative at every wavenumber.
+* The FP32 + learned-closure path after 200 steps, against the FP64
+  TKE-closure run.  Asserted: the mean profile within 1 % (measured 0.24 %)
+  and the plane- and level-summed resolved TKE within 5 % (measured 1.7 %).
+  The per-wavenumber spectra are printed, not asserted.  They differ by up
+  to 42 % at k <= 8 and 87 % overall, so the 5 % spectra target is NOT met.
+  The learned closure dissipates about 2.4x less than the TKE closure (its
+  a-priori mean SGS dissipation is 0.010 against the TKE's 0.024 and the
+  filtered fine run's 0.012).  The seeded noise therefore evolves differently
+  near the walls, and the fluctuation fields decorrelate within 200 steps.
+* 32^3 for 100 steps (the CPU-reference case): FP64 fields within 1e-9 of
+  the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _run(nx, nz, steps, precision, learned=False):
+    import channelflow as C
+    z0 = C.z0_of_retau(395.0)
+    dt = 0.3 / (3.2 * (3 * nx / 2) / (2 * math.pi) + 0.5 / (2.0 / nz))
+    g = C.ChannelFlow(C.RunParams(nx, nx, nz, z0=z0, dt=dt, precision=precision))
+    g.seeded_init(10972, 0.1)
+    net = None
+    if learned:
+        import os
+        w = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                         "paes-channelflow_amd", "weights", "sgs_mlp.cfmlp")
+        net = C.SgsNet(0, w)
+        g.set_closure(net)
+    g.step(steps - 1)
+    for s in (1, 2, 3):
+        g.stage(s, mnout=True)
+    g.sync()
+    st = g.get_state()
+    mb = g.compmn()
+    g.close()
+    if net is not None:
+        net.close()
+    return st, mb
+
+
+def _spectra(st, nz):
+    lv = [nz // 8, nz // 4, nz // 2]
+    h, var, _ = O.calc_spectra([(st["u"], st["v"], st["w"])], lv)
+    return h, var
+
+
+def _report(tag, a, b, ma, mb_, nz):
+    ha, _ = _spectra(a, nz)
+    hb, _ = _spectra(b, nz)
+    r = O.MEAN_NAMES.index
+    out = {k: rel(a[k], b[k]) for k in "uvwep"}
+    out["uxym"] = float(np.max(np.abs(ma[r("uxym")] - mb_[r("uxym")]) / np.abs(mb_[r("uxym")])))
+    out["englez"] = rel(ma[r("englez")], mb_[r("englez")])
+    out["spectra"] = max(float(np.max(np.abs(x[1:] - y[1:]) / np.abs(y[1:]))) for x, y in zip(ha, hb))
+    kmax = ha[0].shape[0] - 1
+    for name, lo, hi in (("spectra_k<=8", 1, 8), ("spectra_k<=kmax/4", 1, kmax // 4),
+                         ("spectra_k<=kmax/2", 1, kmax // 2)):
+        out[name] = max(float(np.max(np.abs(x[lo:hi + 1] - y[lo:hi + 1]) / np.abs(y[lo:hi + 1])))
+                        for x, y in zip(ha, hb))
+    out["englez_ratio"] = float(np.sum(ma[r("englez")]) / np.sum(mb_[r("englez")]))
+    print(tag, {k: f"{v:.2e}" for k, v in out.items()}, flush=True)
+    return out
+
+
+def test_fp32_path_error_per_output_256():
+    import channelflow as C
+    s64, m64 = _run(256, 128, 200, C.CF_PREC_FP64)
+    s32, m32 = _run(256, 128, 200, C.CF_PREC_FP32)
+    e = _report("fp32 vs fp64 after 200 steps:", s32, s64, m32, m64, 128)
+    for k in "uvwep":
+        assert e[k] < 1e-3, (k, e)
+    assert e["uxym"] < 1e-4 and e["englez"] < 1e-3 and e["spectra"] < 1e-3, e
+
+
+def test_learned_path_against_fp64_tke_256():
+    import channelflow as C
+    s64, m64 = _run(256, 128, 200, C.CF_PREC_FP64)
+    sl, ml = _run(256, 128, 200, C.CF_PREC_FP32, learned=True)
+    e = _report("fp32+learned vs fp64 TKE after 200 steps:", sl, s64, ml, m64, 128)
+    assert e["uxym"] < 1e-2, e                     # mean profile within 1 %
+    assert abs(e["englez_ratio"] - 1.0) < 5e-2, e  # total resolved TKE within 5 %
+
+
+def test_fp64_100_steps_32cubed_matches_oracle():
+    import channelflow as C
+    pr = O.Params(32, 32, 32, dt=0.01)
+    u, v, w, e = O.initial_fields(pr)
+    st = O.set_state(pr, u, v, w, e)
+    g = C.ChannelFlow(C.RunParams(32, 32, 32, xl=pr.xl, yl=pr.yl, zl=pr.zl, z0=pr.z0, dt=pr.dt,
+                                  precision=C.CF_PREC_FP64))
+    g.set_state(u, v, w, e)
+    errs = []
+    for it in range(100):
+        O.step(st, pr)
+        g.step(1)
+        if (it + 1) % 25 == 0:
+            gs = g.get_state()
+            errs.append(max(rel(gs[k], O.interior(st, pr, k)) for k in "uvwe"))
+    print("32^3 FP64 max field error at steps 25/50/75/100:", errs, flush=True)
+    assert errs[-1] < 1e-9, errs
+    g.close()
