@@ -124,8 +124,21 @@ int cf_get_scalars(cf_ctx* ctx, double* scalars);
 /* Mean profiles (compmn.F90:1-64) into meanbuf[19*nnz] in the row order of
  * allocate.f90:272-294 (englez, engsbz, divz, uxym, vxym, wxym, uule, vvle,
  * wwle, uwle, vwle, uvle, pple, uusb, vvsb, uvsb, wwsb, uwsb, vwsb), stored
- * meanbuf[row + 19*(iz-1)] like the Fortran meanbuf(19,nnz). */
+ * meanbuf[row + 19*(iz-1)] like the Fortran meanbuf(19,nnz).  Computed on
+ * the device from the band spectra (Parseval); only the profiles cross PCIe. */
 int cf_compmn(cf_ctx* ctx, double* meanbuf);
+
+/* x-y energy spectra of the current state (calc_spectra.f90:139-243 for one
+ * field snapshot, fcounter = 1) at the 1-based levels zlevels[nlev], each in
+ * 1..nnz-1.  u and v average levels zl and zl+1 with their plane means
+ * removed; w is taken at face zl.  Outputs:
+ * - hats[3][nlev][nnx/2+1]: ring spectra of u, v, w, normalised as the
+ *   reference does (bin 0 left unnormalised);
+ * - var[3][nlev];
+ * - lscales[6][nlev]: l111 l112 l221 l222 l331 l332.
+ * Computed on the device from the band spectra. */
+int cf_calc_spectra(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* hats, double* var,
+                    double* lscales);
 
 /* Time nsteps steps with HIP events on the context stream.  Writes the total
  * milliseconds and, when kernel_ms is not NULL, the average duration of each

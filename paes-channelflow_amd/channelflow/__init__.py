@@ -44,6 +44,7 @@ EXPORTS = (
     "cf_mlp_get_norm", "cf_mlp_save", "cf_mlp_load", "cf_mlp_forward", "cf_mlp_reserve",
     "cf_mlp_put_samples", "cf_mlp_loss_grad", "cf_mlp_train", "cf_mlp_reset_optimizer",
     "cf_mlp_last_error", "cf_set_closure", "cf_get_closure_counts", "cf_filter_snapshot",
+    "cf_calc_spectra",
 )
 CF_MLP_NIN = 11
 CF_MLP_NOUT = 6
@@ -124,6 +125,7 @@ def load_library(path: str = LIB_PATH):
     lib.cf_set_closure.argtypes = [vp, vp]
     lib.cf_get_closure_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.cf_filter_snapshot.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp]
+    lib.cf_calc_spectra.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), dp, dp, dp]
     for name in EXPORTS:
         if name not in ("cf_kernel_name", "cf_device_bytes", "cf_last_error", "cf_mlp_last_error"):
             getattr(lib, name).restype = ctypes.c_int
@@ -268,6 +270,21 @@ class ChannelFlow:
 
     def device_bytes(self) -> int:
         return int(self.lib.cf_device_bytes(self._h))
+
+    def calc_spectra(self, zlevels):
+        """calc_spectra.f90 on the device for 1-based levels ``zlevels``:
+        (hats, var, lscales) in the layout of the reference's uhat(r, iz):
+        hats[f] is (nnx/2+1, nlev) for f = u, v, w; var[f] (nlev,);
+        lscales (l111, l112, l221, l222, l331, l332), each (nlev,)."""
+        zl = np.ascontiguousarray(zlevels, dtype=np.int32)
+        nl = zl.size
+        nb = self.params.nnx // 2 + 1
+        h = np.empty((3, nl, nb))
+        var = np.empty((3, nl))
+        ls = np.empty((6, nl))
+        self._check(self.lib.cf_calc_spectra(self._h, nl, zl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             _ptr(h), _ptr(var), _ptr(ls)))
+        return [h[f].T.copy() for f in range(3)], [var[f] for f in range(3)], [ls[q] for q in range(6)]
 
     def set_closure(self, net: "SgsNet | None"):
         """Use the learned closure ``net`` from the next stage on (None: TKE)."""

@@ -23,6 +23,7 @@
 
 #include "../../include/channelflow.h"
 #include "cf_kernels.hpp"
+#include "cf_stats.hpp"
 
 using cf::cx;
 
@@ -86,6 +87,7 @@ struct cf_ctx {
   double* filt_in = nullptr;    // u, v, w, e physical, 4 * nz*ny*nx
   double* filt_out = nullptr;
   size_t filt_out_n = 0;
+  double* stats = nullptr;      // compmn level statistics (cf::NMEAN per level)
 };
 
 namespace {
@@ -137,6 +139,10 @@ struct Engine {
   virtual int physical(cf_ctx* ctx, int which) = 0;
   // d/dx (dir 0) or d/dy (dir 1) of u, v or w (fld 0..2), physical, in ctx->staging
   virtual int physical_deriv(cf_ctx* ctx, int fld, int dir) = 0;
+  // compmn level statistics (cf::NMEAN per level) into the device buffer out
+  virtual int means(cf_ctx* ctx, double* out) = 0;
+  // calc_spectra ring sums for the device level list zl (nlev levels) into out
+  virtual int spectra(cf_ctx* ctx, const int* zl, int nlev, double* out) = 0;
 };
 
 template <typename T, int NX>
@@ -432,6 +438,21 @@ struct EngineT : Engine {
     return CF_OK;
   }
 
+  int means(cf_ctx* ctx, double* out) override {
+    hipLaunchKernelGGL((cf::k_compmn<T>), dim3(ctx->nz), dim3(256), 0, ctx->stream, cptr(ctx->U),
+                       cptr(ctx->V), cptr(ctx->W), rptr(ctx->e), MX, NY, ctx->nz, out);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+
+  int spectra(cf_ctx* ctx, const int* zl, int nlev, double* out) override {
+    const size_t lds = (size_t)(NX / 2 + 4) * sizeof(double);
+    hipLaunchKernelGGL((cf::k_spectra<T>), dim3(nlev, 3), dim3(256), lds, ctx->stream, cptr(ctx->U),
+                       cptr(ctx->V), cptr(ctx->W), zl, MX, NY, NX, out);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+
   int physical_deriv(cf_ctx* ctx, int fld, int dir) override {
     const size_t n = (size_t)ctx->nz * NY * MX;
     void* src = fld == 0 ? ctx->U : (fld == 1 ? ctx->V : ctx->W);
@@ -697,62 +718,90 @@ int cf_get_scalars(cf_ctx* ctx, double* s4) {
 
 int cf_compmn(cf_ctx* ctx, double* meanbuf) {
   if (!ctx || !meanbuf) return CF_E_ARG;
-  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
-  const size_t pl = (size_t)nx * ny;
-  std::vector<double> u((size_t)nz * pl), v(u.size()), w(u.size()), e(u.size());
-  int rc = cf_get_state(ctx, u.data(), v.data(), w.data(), e.data(), nullptr);
-  if (rc != CF_OK) return rc;
-  std::vector<double> red(7 * (size_t)ctx->slots);
-  HIPCHK(hipMemcpy(red.data(), ctx->red, red.size() * sizeof(double), hipMemcpyDeviceToHost));
-  const double xym = 1.0 / (double)pl;
-  auto avg = [&](const double* a) {
-    double s = 0;
-    for (size_t i = 0; i < pl; ++i) s += a[i];
-    return s * xym;
-  };
-  // compmn.F90:18-26: plane means
-  std::vector<double> ux(nz + 1, 0.0), vx(nz + 1, 0.0);
-  for (int i = 0; i < nz; ++i) {
-    ux[i] = avg(&u[i * pl]);
-    vx[i] = avg(&v[i * pl]);
+  const int nz = ctx->nz;
+  HIPCHK(hipSetDevice(ctx->p.device));
+  if (!ctx->stats) {
+    int rc = dalloc(ctx, &ctx->stats, (size_t)cf::NMEAN * nz);
+    if (rc != CF_OK) return rc;
   }
+  int rc = ctx->eng->means(ctx, ctx->stats);
+  if (rc != CF_OK) return rc;
+  std::vector<double> lv((size_t)cf::NMEAN * nz), red(7 * (size_t)ctx->slots);
+  HIPCHK(hipMemcpyAsync(lv.data(), ctx->stats, lv.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(red.data(), ctx->red, red.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  rc = check_flags(ctx);
+  if (rc != CF_OK) return rc;
   std::memset(meanbuf, 0, sizeof(double) * 19 * nz);
-  for (int i = 0; i < nz; ++i) {
+  for (int i = 0; i < nz; ++i) {                                     // compmn.F90:18-62
     double* mb = meanbuf + 19 * (size_t)i;
-    const double* ui = &u[i * pl];
-    const double* vi = &v[i * pl];
-    const double* wi = &w[i * pl];
-    const double* up = (i + 1 < nz) ? &u[(i + 1) * pl] : nullptr;   // u(nnz+1) = 0 (upperbc.f90:189-191)
-    const double* vp = (i + 1 < nz) ? &v[(i + 1) * pl] : nullptr;
-    double uu = 0, vv = 0, ww = 0, uw = 0, vw = 0, uv = 0;
-    for (size_t k = 0; k < pl; ++k) {                               // compmn.F90:29-57
-      const double ul = 0.5 * ((ui[k] - ux[i]) + (up ? up[k] : 0.0) - ux[i + 1]);
-      const double vl = 0.5 * ((vi[k] - vx[i]) + (vp ? vp[k] : 0.0) - vx[i + 1]);
-      uu += ul * ul;
-      vv += vl * vl;
-      ww += wi[k] * wi[k];
-      uw += ul * wi[k];
-      vw += vl * wi[k];
-      uv += ul * vl;
-    }
-    mb[6] = uu * xym;
-    mb[7] = vv * xym;
-    mb[8] = ww * xym;
-    mb[9] = uw * xym;
-    mb[10] = vw * xym;
-    mb[11] = uv * xym;
-    mb[0] = (mb[6] + mb[7] + mb[8]) * 0.5;                           // englez
-    mb[1] = avg(&e[i * pl]);                                         // engsbz
+    const double* l = &lv[(size_t)cf::NMEAN * i];
+    mb[3] = l[0];                                                    // uxym
+    mb[4] = l[1];                                                    // vxym
+    mb[5] = l[2];                                                    // wxym
+    mb[6] = l[3];                                                    // uule
+    mb[7] = l[4];                                                    // vvle
+    mb[8] = l[5];                                                    // wwle
+    mb[9] = l[6];                                                    // uwle
+    mb[10] = l[7];                                                   // vwle
+    mb[11] = l[8];                                                   // uvle
+    mb[0] = 0.5 * (l[3] + l[4] + l[5]);                              // englez
+    mb[1] = l[9];                                                    // engsbz
     mb[2] = red[0 * ctx->slots + i + 1];                             // divz
-    mb[3] = ux[i];
-    mb[4] = vx[i];
-    mb[5] = avg(wi);
     mb[12] = 0.0;                                                    // pple is never computed
     // SGS means uusb vvsb uvsb wwsb uwsb vwsb (allocate.f90:289-294)
-    const int q_of_row[6] = {1, 2, 3, 4, 5, 6};  // red rows: uu vv uv ww uw vw
-    for (int r = 0; r < 6; ++r)
-      mb[13 + r] = ctx->have_sgs ? red[(size_t)q_of_row[r] * ctx->slots + i + 1] : 0.0;
+    for (int r = 0; r < 6; ++r) mb[13 + r] = ctx->have_sgs ? red[(size_t)(r + 1) * ctx->slots + i + 1] : 0.0;
   }
+  return CF_OK;
+}
+
+int cf_calc_spectra(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* hats, double* var,
+                    double* lscales) {
+  if (!ctx || nlev <= 0 || nlev > ctx->nz || !zlevels || !hats || !var || !lscales) return CF_E_ARG;
+  for (int i = 0; i < nlev; ++i)
+    if (zlevels[i] < 1 || zlevels[i] > ctx->nz - 1) {
+      ctx->err = "calc_spectra levels must lie in 1..nnz-1";
+      return CF_E_ARG;
+    }
+  HIPCHK(hipSetDevice(ctx->p.device));
+  const int nx = ctx->nx, ny = ctx->ny, nb = nx / 2 + 1;
+  const size_t nout = (size_t)nlev * 3 * (nb + 3);
+  int* dz = nullptr;
+  double* dout = nullptr;
+  HIPCHK(hipMallocAsync((void**)&dz, nlev * sizeof(int), ctx->stream));
+  HIPCHK(hipMallocAsync((void**)&dout, nout * sizeof(double), ctx->stream));
+  HIPCHK(hipMemcpyAsync(dz, zlevels, nlev * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  int rc = ctx->eng->spectra(ctx, dz, nlev, dout);
+  std::vector<double> o(nout);
+  if (rc == CF_OK)
+    HIPCHK(hipMemcpyAsync(o.data(), dout, nout * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipFreeAsync(dz, ctx->stream));
+  HIPCHK(hipFreeAsync(dout, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (rc != CF_OK) return rc;
+  // ring counts of one (field, level): calc_spectra.f90:150-166 over the r2c
+  // half plane kx = 0..nx/2 (weight 2 for 0 < kx < nx/2), every ky
+  std::vector<double> cnt(nb, 0.0);
+  for (int kx = 0; kx <= nx / 2; ++kx)
+    for (int j = 0; j < ny; ++j) {
+      const int ky = j <= nx / 2 ? j : j - nx;
+      const int r = (int)std::lrint(std::sqrt((double)(kx * kx + ky * ky)));
+      if (r <= nx / 2) cnt[r] += (kx >= 1 && kx <= nx / 2 - 1) ? 2.0 : 1.0;
+    }
+  for (int f = 0; f < 3; ++f)
+    for (int l = 0; l < nlev; ++l) {
+      const double* b = &o[((size_t)l * 3 + f) * (nb + 3)];
+      double* h = hats + ((size_t)f * nlev + l) * nb;
+      h[0] = b[0];                                                   // the r = 0 bin is not normalised
+      for (int r = 1; r < nb; ++r)                                   // calc_spectra.f90:238-243
+        h[r] = b[r] * PI * ((r + 0.5) * (r + 0.5) - (r - 0.5) * (r - 0.5)) / cnt[r];
+      const double v = b[nb + 2];
+      var[(size_t)f * nlev + l] = v;
+      lscales[(size_t)(2 * f) * nlev + l] = b[nb] * PI / v;          // l_f1
+      lscales[(size_t)(2 * f + 1) * nlev + l] = b[nb + 1] * PI / v;  // l_f2
+    }
   return CF_OK;
 }
 

@@ -174,4 +174,15 @@ def test_fp64_128x64_fields_divergence_means_spectra():
     for f in range(3):
         assert np.allclose(hg[f], ho[f], rtol=1e-8, atol=1e-30), f
         assert np.allclose(vg[f], vo[f], rtol=1e-8), f
+    # calc_spectra on the device, at the reference's own level list (calc_spectra.f90:32)
+    zl = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 18, 24, 30, 36, 48]
+    hd, vd, ld = g.calc_spectra(zl)
+    ho, vo, lo = O.calc_spectra([(ref["u"], ref["v"], ref["w"])], zl)
+    for f in range(3):
+        scale = np.abs(ho[f][1:]).max()
+        assert np.allclose(hd[f][1:], ho[f][1:], rtol=1e-8, atol=1e-12 * scale), f
+        assert np.allclose(hd[f][0], ho[f][0], rtol=1e-6, atol=1e-20), f
+        assert np.allclose(vd[f], vo[f], rtol=1e-8), f
+    for q in range(6):
+        assert np.allclose(ld[q], lo[q], rtol=1e-8), q
     g.close()
