@@ -56,7 +56,22 @@ def kernel_bytes(name, n, s, istage=2):
         "k_press": 15 * n * s + 8 * n,
         # x-spectral re, e, re in; e, re, x-spectral e out
         "k_erows": (6 if istage > 1 else 5) * n * s,
+        # small-grid closure: U, V, W -> u, v, w, uy, vy, wy small rows (6 column
+        # transforms, 3 i ky products), each a band in and a band out
+        "k_sgs_cols_in": 18 * n * s,
+        # 6 small rows + physical e in; G and F^y rows of u, v, w out
+        "k_sgs_rows": 13 * n * s,
+        # 6 column transforms (band in, band out) and R += G + i ky F^y (3 in, 1 out)
+        "k_sgs_cols_out": 24 * n * s,
     }.get(name)
+
+BASIS = {
+    "k_rows": "24 x N x 4 B per launch (3/2-grid x-spectral rows in and out, once each)",
+    "k_rows_mlp": "24 x N x 4 B per launch (3/2-grid x-spectral rows in and out, once each; "
+                  "the network's work is on-chip)",
+    "k_sgs_rows": "13 x N x 4 B per launch (6 small x-spectral rows and the physical e in, "
+                  "6 divergence rows out; the network's work is on-chip)",
+}
 
 # HBM bytes per dispatch from the FETCH_SIZE and WRITE_SIZE counters (KiB per
 # dispatch x 1024) for the 256x256x128 FP32 workload.  FETCH_SIZE counts
@@ -82,6 +97,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain path")
     ap.add_argument("--no-tke32", action="store_true", help="skip the FP32 TKE-closure path")
     ap.add_argument("--weights", default=WEIGHTS, help="learned closure network file")
+    ap.add_argument("--closure-grid", choices=("dealiased", "small"), default="dealiased",
+                    help="evaluate the network on the 3/2 grid or on the nnx x nny grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=0,
                     help="levels of the CPU-baseline sample (default: the full nz)")
@@ -107,6 +124,8 @@ def run_gpu(args, precision_name, learned=False):
     if learned:
         net = C.SgsNet(0, args.weights)
         g.set_closure(net)
+        g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL if args.closure_grid == "small"
+                           else C.CF_CLOSURE_GRID_DEALIASED)
     if args.warmup > 0:
         g.step(args.warmup)
     g.sync()
@@ -192,9 +211,9 @@ def main():
     achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
     per_kernel_gbs = {k: kernel_bytes(k, pts, 4) / (t * 1e-3) / 1e9
                       for k, t in kern.items() if kernel_bytes(k, pts, 4) and t > 0}
-    # the learned path runs the row kernel with the network inside (k_rows_mlp):
-    # same row inputs and outputs, so the same algorithmic bytes
-    dom_label = "k_rows_mlp" if dom == "k_rows" else dom
+    # on the 3/2 grid the learned path runs the row kernel with the network
+    # inside (k_rows_mlp): same row inputs and outputs, same algorithmic bytes
+    dom_label = "k_rows_mlp" if dom == "k_rows" and args.closure_grid == "dealiased" else dom
     traffic = MEASURED_TRAFFIC.get((dom_label, args.nx, args.nz, "fp32"))
     line = {
         "metric": "grid-pt RK3 steps/s @256^2x128: FP32+learned SGS vs FP64 plain; % HBM peak",
@@ -217,6 +236,7 @@ def main():
             "grid": [args.nx, args.nx, args.nz],
             "precision": "fp32 fields/FFT/SGS, fp64 pressure, bf16 MFMA network with fp32 accumulation",
             "learned_sgs": True,
+            "closure_grid": args.closure_grid,
             "closure_cells_accepted": r32["accepted"][0] / max(r32["accepted"][1], 1),
         },
         "roofline": {
@@ -229,8 +249,8 @@ def main():
             "kernel": dom_label,
             "bytes_per_launch": dom_bytes,
             "ms_per_launch": kern[dom],
-            "basis": f"{dom_label}: 24 x N x 4 B per launch (3/2-grid x-spectral rows in and out, "
-                     "once each) / mean launch time from HIP events on the library stream",
+            "basis": f"{dom_label}: {BASIS.get(dom_label, 'algorithmic bytes per launch')} "
+                     "/ mean launch time from HIP events on the library stream",
         },
         "step_roofline": {
             "achieved": step_achieved,

@@ -144,7 +144,9 @@ int cf_calc_spectra(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* h
  * milliseconds and, when kernel_ms is not NULL, the average duration of each
  * kernel family of one step measured with events bracketing every launch
  * (kernel_ms[CF_NKERN]; the per-kernel pass runs after the total timing). */
-#define CF_NKERN 8
+/* k_wall k_col_synth k_rows k_col_anal | k_sgs_cols_in k_sgs_rows k_sgs_cols_out
+ * (the small-grid closure: zero time otherwise) | k_press k_erows k_reduce_part */
+#define CF_NKERN 10
 int cf_time_steps(cf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms);
 
 /* Name of kernel family k of cf_time_steps (0 <= k < CF_NKERN). */
@@ -222,6 +224,19 @@ const char* cf_mlp_last_error(const cf_mlp* m);
  * current parameters are copied); NULL restores the TKE closure.  Needs
  * nnx in {128, 256, 512}. */
 int cf_set_closure(cf_ctx* ctx, const cf_mlp* m);
+
+/* Where the network is evaluated.
+ * - CF_CLOSURE_GRID_DEALIASED (default): at the 3/2-grid cell centres,
+ *   inside the row-tendency sweep, where the TKE stresses of compsgs1 live.
+ * - CF_CLOSURE_GRID_SMALL: at the nnx x nny cell centres, the grid the
+ *   training features are computed on, with the TKE fallback on the same
+ *   grid.  The stress divergence is added to the compressed tendencies, and
+ *   the wall-model faces stay on the 3/2 grid.  It needs 2.25x fewer network
+ *   evaluations.
+ * Applies from the next stage. */
+#define CF_CLOSURE_GRID_DEALIASED 0
+#define CF_CLOSURE_GRID_SMALL 1
+int cf_set_closure_grid(cf_ctx* ctx, int32_t grid);
 
 /* Cells where the network's stress was used, and cells evaluated, since the
  * previous call (then reset): counts[2]. */

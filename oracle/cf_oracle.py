@@ -497,6 +497,8 @@ def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
     st.re[ns] = compress(reb[: n - 1], pr.nnx, pr.nny)
     st.rw[n + OF] = 0.0                                             # comp1.f90:174-177
     st.re[n + OF] = 0.0
+    if pr.closure is not None and hasattr(pr.closure, "post"):
+        pr.closure.post(st, pr, mnout and istage == 1)              # small-grid learned stresses
 
 
 

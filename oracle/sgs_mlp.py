@@ -269,3 +269,109 @@ def deviatoric(tau):
     for i in range(3):
         d[..., i] -= tr
     return d
+
+
+# ----------------------------------------------------------------------------
+# The closure evaluated on the nnx x nny grid ("small grid" mode)
+# ----------------------------------------------------------------------------
+def small_grid_stresses(st, pr: O.Params, net):
+    """Network and TKE stresses at the cell centres of the small grid, from the
+    stage-start state (after lower/upper).  Returns (cell, faces, ok):
+    cell = (t11, t22, t12, t33) at levels 1..nnz; faces = (t13, t23) at
+    faces 1..nnz-1; ok = accepted cells."""
+    n = pr.nnz
+    OF = O.OF
+    xl, yl = pr.xl, pr.yl
+    u = st.u[0 + OF: n + 2 + OF]                     # levels 0..nnz+1
+    v = st.v[0 + OF: n + 2 + OF]
+    w = st.w[0 + OF: n + 1 + OF]                     # faces 0..nnz
+    e = st.e[0 + OF: n + 1 + OF].copy()              # faces 0..nnz
+    e[0] = e[1]                                      # lower (lowerbc.f90:13)
+    e[n] = e[n - 1]                                  # upper (upperbc.f90:197-201)
+    uk, vk = u[1:n + 1], v[1:n + 1]
+    ux, uy = O.xderiv(uk, xl), O.yderiv(uk, yl)
+    vx, vy = O.xderiv(vk, xl), O.yderiv(vk, yl)
+    uz = (u[2:n + 2] - u[0:n]) * pr.todzm
+    vz = (v[2:n + 2] - v[0:n]) * pr.todzm
+    wxf, wyf = O.xderiv(w, xl), O.yderiv(w, yl)
+    wx, wy = 0.5 * (wxf[:-1] + wxf[1:]), 0.5 * (wyf[:-1] + wyf[1:])
+    wz = (w[1:] - w[:-1]) * pr.dzm
+    zm = (np.arange(1, n + 1) - 0.5) * pr.dz
+    wall = np.minimum(zm, pr.zl - zm)[:, None, None] * np.ones_like(uz)
+    feats = np.stack([ux, uy, uz, vx, vy, vz, wx, wy, wz, np.full_like(uz, pr.dsl), wall], axis=-1)
+    K = np.minimum(0.1 * pr.alk * np.sqrt(e), pr.xkmax)          # complen.f90:88-89, faces 0..nnz
+    xs = K[:-1] + K[1:]
+    tke = (-xs * ux, -xs * vy, -xs * 0.5 * (uy + vx), -xs * wz)   # compsgs1.f90:144-151
+    uzf = (u[2:n + 1] - u[1:n]) * pr.dzm                          # faces 1..nnz-1
+    vzf = (v[2:n + 1] - v[1:n]) * pr.dzm
+    tke13 = -K[1:n] * (uzf + wxf[1:n])                            # compsgs1.f90:152-153
+    tke23 = -K[1:n] * (vzf + wyf[1:n])
+    tau = net.stresses(feats)
+    ok = accept(tau, feats)
+    tr = (tau[..., 0] + tau[..., 1] + tau[..., 2]) / 3.0
+    cell = (np.where(ok, tau[..., 0] - tr, tke[0]), np.where(ok, tau[..., 1] - tr, tke[1]),
+            np.where(ok, tau[..., 3], tke[2]), np.where(ok, tau[..., 2] - tr, tke[3]))
+    okf = ok[:-1] & ok[1:]
+    faces = (np.where(okf, 0.5 * (tau[:-1, ..., 4] + tau[1:, ..., 4]), tke13),
+             np.where(okf, 0.5 * (tau[:-1, ..., 5] + tau[1:, ..., 5]), tke23))
+    return cell, faces, ok
+
+
+class LearnedClosureSmall:
+    """The closure hook of cf_oracle.comp1 in small-grid mode.
+
+    It does three things:
+    - The interior SGS momentum stresses of the 3/2-grid sweep are switched
+      off; the wall-model faces 0 and nnz stay there.
+    - The stresses of ``small_grid_stresses`` are differentiated on the small
+      grid: spectral d/dx, d/dy; dz differences with zero interior flux at
+      the walls.
+    - The result is band-limited (Nyquist modes dropped) and added to the
+      compressed tendencies; the plane mean of the w tendency is removed, as
+      comp1.f90:172 does for rw.
+    """
+
+    def __init__(self, net: Network):
+        self.net = net
+        self.last_accept = None
+        self._r = None
+        self._means = None
+
+    def apply(self, F, pr: O.Params, st):
+        n = pr.nnz
+        OF = O.OF
+        xl, yl = pr.xl, pr.yl
+        (c11, c22, c12, c33), (f13, f23), ok = small_grid_stresses(st, pr, self.net)
+        self.last_accept = ok
+        shp = c11.shape[1:]
+        zero = np.zeros((1,) + shp)
+        F13 = np.concatenate([zero, f13, zero])                  # faces 0..nnz
+        F23 = np.concatenate([zero, f23, zero])
+        ru = -O.xderiv(c11, xl) - O.yderiv(c12, yl) - (F13[1:] - F13[:-1]) * pr.dzm
+        rv = -O.xderiv(c12, xl) - O.yderiv(c22, yl) - (F23[1:] - F23[:-1]) * pr.dzm
+        rw = -O.xderiv(f13, xl) - O.yderiv(f23, yl) - (c33[1:] - c33[:-1]) * pr.dzm
+        rw = O.deal(rw)
+        rw -= rw.mean(axis=(-2, -1), keepdims=True)
+        self._r = (O.deal(ru), O.deal(rv), rw)
+        mean = lambda a: a.mean(axis=(-2, -1))                   # noqa: E731
+        self._means = dict(uusb=mean(c11), vvsb=mean(c22), uvsb=mean(c12), wwsb=mean(c33),
+                           uwsb=np.concatenate([mean(f13), [0.0]]), vwsb=np.concatenate([mean(f23), [0.0]]))
+        # the 3/2-grid sweep keeps only the wall stresses
+        for k in ("tauuu", "tauvv", "tauuv"):
+            F[k] = np.zeros_like(F[k])
+        F["tauww"] = np.zeros_like(F["tauww"])
+        for k in ("tauuw", "tauvw"):
+            t = F[k].copy()
+            t[1 + OF: n + OF] = 0.0
+            F[k] = t
+
+    def post(self, st, pr: O.Params, means: bool):
+        n = pr.nnz
+        OF = O.OF
+        ru, rv, rw = self._r
+        st.ru[1 + OF: n + 1 + OF] += ru
+        st.rv[1 + OF: n + 1 + OF] += rv
+        st.rw[1 + OF: n + OF] += rw
+        if means:                                  # the SGS means of compsgs3.f90:236-243
+            for k, v in self._means.items():
+                st.sgs[k] = st.sgs[k] + v

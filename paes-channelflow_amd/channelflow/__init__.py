@@ -32,7 +32,7 @@ CF_E_ECLAMP = -5
 CF_E_NODEV = -6
 CF_PREC_FP64 = 0
 CF_PREC_FP32 = 1
-CF_NKERN = 8
+CF_NKERN = 10
 
 EXPORTS = (
     "cf_create", "cf_destroy", "cf_set_state", "cf_seeded_init", "cf_get_state", "cf_get_tendency",
@@ -44,8 +44,10 @@ EXPORTS = (
     "cf_mlp_get_norm", "cf_mlp_save", "cf_mlp_load", "cf_mlp_forward", "cf_mlp_reserve",
     "cf_mlp_put_samples", "cf_mlp_loss_grad", "cf_mlp_train", "cf_mlp_reset_optimizer",
     "cf_mlp_last_error", "cf_set_closure", "cf_get_closure_counts", "cf_filter_snapshot",
-    "cf_calc_spectra",
+    "cf_calc_spectra", "cf_set_closure_grid",
 )
+CF_CLOSURE_GRID_DEALIASED = 0
+CF_CLOSURE_GRID_SMALL = 1
 CF_MLP_NIN = 11
 CF_MLP_NOUT = 6
 CF_MLP_NPARAM = 9478
@@ -126,6 +128,7 @@ def load_library(path: str = LIB_PATH):
     lib.cf_get_closure_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.cf_filter_snapshot.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp]
     lib.cf_calc_spectra.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), dp, dp, dp]
+    lib.cf_set_closure_grid.argtypes = [vp, ctypes.c_int32]
     for name in EXPORTS:
         if name not in ("cf_kernel_name", "cf_device_bytes", "cf_last_error", "cf_mlp_last_error"):
             getattr(lib, name).restype = ctypes.c_int
@@ -289,6 +292,10 @@ class ChannelFlow:
     def set_closure(self, net: "SgsNet | None"):
         """Use the learned closure ``net`` from the next stage on (None: TKE)."""
         self._check(self.lib.cf_set_closure(self._h, net._h if net is not None else None))
+
+    def set_closure_grid(self, grid: int):
+        """CF_CLOSURE_GRID_DEALIASED (3/2 grid) or CF_CLOSURE_GRID_SMALL (nnx x nny)."""
+        self._check(self.lib.cf_set_closure_grid(self._h, grid))
 
     def closure_counts(self):
         """(cells that used the network's stress, cells evaluated) since the last call."""

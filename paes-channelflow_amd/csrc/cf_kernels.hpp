@@ -73,6 +73,7 @@ struct Consts {
   double xymb, zody, vk, zl;
   double dsl;                 // filter width (dx dy dz)^(1/3), les.F90:118
   int nz, istage, mnout, zchunk;
+  int sgs_momentum;           // 0: interior SGS momentum stresses off (small-grid closure)
 };
 
 // Sum of NQ per-thread values over the workgroup; results land in out[q] of
@@ -415,6 +416,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
   const T c_lo = (T)a.ws->c_lo, c_hi = (T)a.ws->c_hi;
   const bool mn = c.mnout != 0;
+  const bool sgs_mom = c.sgs_momentum != 0;
   int clampflag = 0;
 
   auto xkk_of = [&](T e) {
@@ -502,13 +504,13 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     {
       const T xm = xkk_of(em), x0 = xkk_of(e0), xp = xkk_of(ep);
       const T xs = xm + x0;
-      const T tuu = -xs * ux;                                     // compsgs1.f90:144-146
-      const T tvv = -xs * vy;
-      const T tuv = -xs * (uy + vx) * T(0.5);
+      T tuu = -xs * ux;                                           // compsgs1.f90:144-146
+      T tvv = -xs * vy;
+      T tuv = -xs * (uy + vx) * T(0.5);
       const T wz0 = (w0 - wm) * dzm;                              // derivs: wz at level iz
       const T wzp = (wp - w0) * dzm;
-      const T tww0 = -xs * wz0;                                   // compsgs1.f90:129,151
-      const T twwp = -(x0 + xp) * wzp;
+      T tww0 = -xs * wz0;                                         // compsgs1.f90:129,151
+      T twwp = -(x0 + xp) * wzp;
       T tuw0, tvw0, tuwm, tvwm;
       if (top) {                                                  // upperbc.f90:304-305
         tuw0 = c_hi * (u0 + ugal);
@@ -523,6 +525,11 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
       } else {
         tuwm = -xm * ((u0 - um) * dzm + wxm);
         tvwm = -xm * ((v0 - vm) * dzm + wym);
+      }
+      if (!sgs_mom) {            // the small-grid closure supplies them; the walls stay here
+        tuu = tvv = tuv = tww0 = twwp = T(0);
+        if (!top) tuw0 = tvw0 = T(0);
+        if (iz != 1) tuwm = tvwm = T(0);
       }
       // compadv1.f90:19-51 and compsgs3.f90:205-224
       fa[0] = -(u0 * u0 + tuu);
@@ -901,6 +908,340 @@ template <typename T, int NX>
 __global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp::Net* __restrict__ gnet,
                                                          unsigned long long* counts) {
   rows_mlp_body<T, NX>(a, gnet, counts);
+}
+
+// ---------------------------------------------------------------------------
+// Learned closure on the nnx x nny grid (no reference counterpart).
+//
+// The network and its TKE fallback run at the cell centres of the small grid.
+// The inputs and the fallback formulas are those of k_rows_mlp, with K from
+// the physical e at the faces (compsgs1.f90:144-153).  The row kernel emits
+// the stress-divergence contributions to the tendencies as x-spectral rows:
+//   G_u = i kx F^x + H^ and F^y, per component, with
+//   u: Fx = -t11, Fy = -t12, H = -(t13(iz) - t13(iz-1))/dz;
+//   v: Fx = -t12, Fy = -t22, H = -(t23(iz) - t23(iz-1))/dz;
+//   w (face iz): Fx = -t13(iz), Fy = -t23(iz), H = -(t33(iz+1) - t33(iz))/dz.
+// The wall faces 0 and nnz carry no stress here: the 3/2-grid sweep keeps
+// the wall model (k_rows with sgs_momentum == 0).  k_sgs_add then adds
+// G + i ky F^y, y-transformed, to the comp1 tendencies.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SgsRowArgs {
+  const cx<T>* S;          // u, v, w, uy, vy, wy as x-spectral small rows [6][slot][NY][MX]
+  size_t sstride;
+  cx<T>* O;                // G_u, Fy_u, G_v, Fy_v, G_w, Fy_w [6][slot][NY][MX]
+  const T* e;              // physical e [slot][NY][NX]
+  const T* kxp;
+  const cx<T>* twNX;
+  double* sred;            // SGS means [6][slots] (added atomically) when mnout
+  int slots;
+  unsigned long long* counts;
+  Consts c;
+};
+
+// Half spectra (k < MX) of two real rows -> the length-L spectrum whose
+// inverse DFT is a + i b; modes MX .. L-MX are zero.
+template <typename T, int L, int MX>
+__device__ __forceinline__ void fill_pair_len(cx<T>* z, const cx<T>* sa, bool da, const cx<T>* sb,
+                                              bool db, const T* __restrict__ kxp, int t, int nt) {
+  for (int k = t; k < MX; k += nt) {
+    cx<T> av = {0, 0}, bv = {0, 0};
+    if (sa) av = sa[k];
+    if (sb) bv = sb[k];
+    const T kx = kxp[k];
+    if (da) av = cx<T>{-kx * av.y, kx * av.x};
+    if (db) bv = cx<T>{-kx * bv.y, kx * bv.x};
+    if (k == 0) {
+      z[0] = cx<T>{av.x, bv.x};
+    } else {
+      z[pidx(k)] = cx<T>{av.x - bv.y, av.y + bv.x};
+      z[pidx(L - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
+    }
+  }
+  for (int i = MX + t; i <= L - MX; i += nt) z[pidx(i)] = cx<T>{0, 0};
+}
+
+template <typename T, int NX>
+struct SmallRowFFT {
+  static constexpr int L = NX, MX = NX / 2, NT = NX;
+  static constexpr int NW = NX / 64;
+  static constexpr int TPW = (6 + NW - 1) / NW;
+};
+
+template <typename T, int NX, int NP>
+__device__ __forceinline__ void synth_small(cx<T>* buf, const cx<T>* const* sa, const bool* da,
+                                            const cx<T>* const* sb, const bool* db, T* ra, T* rb,
+                                            const T* __restrict__ kxp, const cx<T>* tw, int tid) {
+  using F = SmallRowFFT<T, NX>;
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int qq = 0; qq < F::TPW; ++qq) {
+    const int q = w * F::TPW + qq;
+    if (q < NP) fill_pair_len<T, NX, F::MX>(buf + q * padlen(NX), sa[q], da[q], sb[q], db[q], kxp, lane, 64);
+  }
+  wave_sync();
+  if (w * F::TPW < NP) fft_run<T, NX, true, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NX), tw, lane);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const cx<T> v = buf[q * padlen(NX) + pidx(tid)];
+    ra[q] = v.x;
+    rb[q] = v.y;
+  }
+}
+
+template <typename T, int NX, int NP, typename Emit>
+__device__ __forceinline__ void anal_small(cx<T>* buf, const T* xa, const T* xb, const cx<T>* tw, int tid,
+                                           Emit emit) {
+  using F = SmallRowFFT<T, NX>;
+  constexpr int MX = F::MX;
+  const T s = T(0.5) / T(NX);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NP; ++q) buf[q * padlen(NX) + pidx(tid)] = cx<T>{xa[q], xb[q]};
+  __syncthreads();
+  const int w = tid >> 6, lane = tid & 63;
+  if (w * F::TPW < NP) {
+    fft_run<T, NX, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NX), tw, lane);
+#pragma unroll
+    for (int qq = 0; qq < F::TPW; ++qq) {
+      const int q = w * F::TPW + qq;
+      if (q < NP)
+        for (int k = lane; k < MX; k += 64) {
+          const cx<T> zk = buf[q * padlen(NX) + pidx(k)];
+          const cx<T> zn = buf[q * padlen(NX) + pidx(k == 0 ? 0 : NX - k)];
+          emit(q, k, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s}, cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
+        }
+    }
+  }
+}
+
+// Two waves per SIMD: left to itself the compiler takes 261 registers for a
+// 256-thread block (one wave per SIMD, MFMA and LDS latencies exposed); three
+// waves spill.
+template <typename T, int NX>
+__global__ void __launch_bounds__(NX) __attribute__((amdgpu_waves_per_eu(2)))
+k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
+  using F = SmallRowFFT<T, NX>;
+  static_assert(NX % 64 == 0, "whole wavefronts of points");
+  constexpr int MX = F::MX, NT = F::NT, NY = NX;
+  __shared__ cx<T> buf[6 * padlen(NX)];
+  __shared__ cx<T> tw[NX];
+  __shared__ mlp::Net net;
+  __shared__ double red[6 * (NX / 64)];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int y = blockIdx.x;
+  const Consts& c = a.c;
+  const int nz = c.nz;
+  const int z0 = 1 + blockIdx.y * c.zchunk;
+  int z1 = z0 + c.zchunk;
+  if (z1 > nz + 1) z1 = nz + 1;
+  const size_t plane = (size_t)NY * MX;
+  const size_t rowoff = (size_t)y * MX;
+  const size_t pplane = (size_t)NY * NX;
+  enum { SU = 0, SV, SW, SUY, SVY, SWY };
+  auto row = [&](int fld, int L) -> const cx<T>* {
+    return a.S + (size_t)fld * a.sstride + (size_t)L * plane + rowoff;
+  };
+  auto lvl = [&](int fld, int L) -> const cx<T>* { return (L >= 1 && L <= nz) ? row(fld, L) : nullptr; };
+  auto face = [&](int fld, int L) -> const cx<T>* { return (L >= 1 && L <= nz - 1) ? row(fld, L) : nullptr; };
+  // e at face f, with e(f <= 0) = e(1) and e(f >= nnz) = e(nnz-1) (lower/upper)
+  auto e_at = [&](int f) -> T {
+    const int g = f <= 0 ? 1 : (f >= nz ? nz - 1 : f);
+    return a.e[(size_t)g * pplane + (size_t)y * NX + tid];
+  };
+  load_twiddles(tw, a.twNX, NX, tid, NT);
+  for (int q = tid; q < (int)(sizeof(mlp::Net) / 16); q += NT) ((uint4*)&net)[q] = ((const uint4*)gnet)[q];
+
+  const T alk = (T)c.alk, xkmax = (T)c.xkmax;
+  const T dzm = (T)c.dzm, todzm = (T)c.todzm;
+  const float dslf = (float)c.dsl, dzf = (float)c.dz, zlf = (float)c.zl;
+  const bool mn = c.mnout != 0;
+  unsigned long long nacc = 0;
+  auto xkk_of = [&](T e) {
+    T v = T(0.1) * alk * sqrt(e);
+    return v < xkmax ? v : xkmax;   // complen.f90:89
+  };
+  auto closure = [&](int k, const float (&g)[9], float (&t)[6]) -> bool {
+    const float zc = ((float)k - 0.5f) * dzf;
+    const float f[mlp::NIN] = {g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], dslf,
+                               fminf(zc, zlf - zc)};
+    mlp::infer(&net, f, t, lane);
+    return mlp::accept(t, g) && k >= 1 && k <= nz;
+  };
+
+  T um, u0, up, upp, vm, v0, vp, vpp, wm, w0, wp, em, e0, ep;
+  T ux0, uy0, vx0, vy0, ux1, uy1, vx1, vy1;
+  T wxm, wym, wx0, wy0, wxp, wyp;
+  float t0[6];
+  bool ok0;
+  T fuw = 0, fvw = 0;                       // final t13, t23 at face iz-1
+  const int zs = z0 - 1;
+  {
+    const cx<T>* sa[6];
+    const cx<T>* sb[6];
+    const bool da[6] = {false, false, false, false, false, false};
+    const bool db[6] = {false, false, false, true, true, false};
+    T ra[6], rb[6];
+    sa[0] = lvl(SU, zs - 1); sb[0] = lvl(SV, zs - 1);
+    sa[1] = lvl(SU, zs);     sb[1] = lvl(SV, zs);
+    sa[2] = lvl(SU, zs + 1); sb[2] = lvl(SV, zs + 1);
+    sa[3] = face(SW, zs - 1); sb[3] = face(SW, zs - 1);     // w, wx at face zs-1
+    sa[4] = face(SW, zs);     sb[4] = face(SW, zs);         // w, wx at face zs
+    sa[5] = face(SWY, zs - 1); sb[5] = face(SWY, zs);       // wy at faces zs-1, zs
+    synth_small<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+    um = ra[0]; vm = rb[0];
+    u0 = ra[1]; v0 = rb[1];
+    up = ra[2]; vp = rb[2];
+    wm = ra[3]; wxm = rb[3];
+    w0 = ra[4]; wx0 = rb[4];
+    wym = ra[5]; wy0 = rb[5];
+    __syncthreads();
+    const bool da2[2] = {true, false};
+    const bool db2[2] = {true, false};
+    sa[0] = lvl(SU, zs);  sb[0] = lvl(SV, zs);               // ux, vx
+    sa[1] = lvl(SUY, zs); sb[1] = lvl(SVY, zs);              // uy, vy
+    synth_small<T, NX, 2>(buf, sa, da2, sb, db2, ra, rb, a.kxp, tw, tid);
+    ux0 = ra[0]; vx0 = rb[0];
+    uy0 = ra[1]; vy0 = rb[1];
+    __syncthreads();
+    em = e_at(zs - 1);
+    e0 = e_at(zs);
+    const float g[9] = {(float)ux0, (float)uy0, (float)((up - um) * todzm), (float)vx0, (float)vy0,
+                        (float)((vp - vm) * todzm), (float)(T(0.5) * (wxm + wx0)),
+                        (float)(T(0.5) * (wym + wy0)), (float)((w0 - wm) * dzm)};
+    ok0 = closure(zs, g, t0);
+  }
+
+  for (int iz = zs; iz < z1; ++iz) {
+    {
+      T ra[5], rb[5];
+      const cx<T>* sa[5];
+      const cx<T>* sb[5];
+      const bool da[5] = {false, true, false, false, false};
+      const bool db[5] = {false, true, false, true, false};
+      sa[0] = lvl(SU, iz + 2);  sb[0] = lvl(SV, iz + 2);
+      sa[1] = lvl(SU, iz + 1);  sb[1] = lvl(SV, iz + 1);      // ux, vx
+      sa[2] = lvl(SUY, iz + 1); sb[2] = lvl(SVY, iz + 1);     // uy, vy
+      sa[3] = face(SW, iz + 1); sb[3] = face(SW, iz + 1);     // w, wx
+      sa[4] = face(SWY, iz + 1); sb[4] = nullptr;             // wy
+      synth_small<T, NX, 5>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      upp = ra[0]; vpp = rb[0];
+      ux1 = ra[1]; vx1 = rb[1];
+      uy1 = ra[2]; vy1 = rb[2];
+      wp = ra[3]; wxp = rb[3];
+      wyp = ra[4];
+    }
+    ep = e_at(iz + 1);
+    float t1[6];
+    bool ok1;
+    {
+      const float g[9] = {(float)ux1, (float)uy1, (float)((upp - u0) * todzm), (float)vx1, (float)vy1,
+                          (float)((vpp - v0) * todzm), (float)(T(0.5) * (wx0 + wxp)),
+                          (float)(T(0.5) * (wy0 + wyp)), (float)((wp - w0) * dzm)};
+      ok1 = closure(iz + 1, g, t1);
+    }
+    const bool top = (iz == nz);
+    const T xm = xkk_of(em), x0 = xkk_of(e0), xp = xkk_of(ep);
+    // final stresses of face iz (the wall faces carry none here)
+    T f13 = 0, f23 = 0;
+    if (iz >= 1 && iz <= nz - 1) {
+      if (ok0 && ok1) {
+        f13 = (T)(0.5f * (t0[4] + t1[4]));
+        f23 = (T)(0.5f * (t0[5] + t1[5]));
+      } else {                                                    // compsgs1.f90:152-153
+        f13 = -x0 * ((up - u0) * dzm + wx0);
+        f23 = -x0 * ((vp - v0) * dzm + wy0);
+      }
+    }
+    if (iz >= z0) {
+      const T xs = xm + x0;
+      T c11 = -xs * ux0, c22 = -xs * vy0, c12 = -xs * (uy0 + vx0) * T(0.5);   // compsgs1.f90:144-146
+      T c33 = -xs * (w0 - wm) * dzm;
+      T c33p = -(x0 + xp) * (wp - w0) * dzm;
+      if (ok0) {
+        const float tr = (t0[0] + t0[1] + t0[2]) * (1.0f / 3.0f);
+        c11 = (T)(t0[0] - tr);
+        c22 = (T)(t0[1] - tr);
+        c12 = (T)t0[3];
+        c33 = (T)(t0[2] - tr);
+      }
+      if (ok1) c33p = (T)(t1[2] - (t1[0] + t1[1] + t1[2]) * (1.0f / 3.0f));
+      const T fm13 = (iz == 1) ? T(0) : fuw, fm23 = (iz == 1) ? T(0) : fvw;
+      T fa[4], fb[4];
+      fa[0] = -c11; fb[0] = -(f13 - fm13) * dzm;                  // (Fx_u, H_u)
+      fa[1] = -c12; fb[1] = -(f23 - fm23) * dzm;                  // (Fy_u = Fx_v, H_v)
+      fa[2] = top ? T(0) : -f13;                                  // (Fx_w, H_w) at face iz
+      fb[2] = top ? T(0) : -(c33p - c33) * dzm;
+      fa[3] = -c22; fb[3] = top ? T(0) : -f23;                    // (Fy_v, Fy_w)
+      const unsigned long long bal = __ballot(ok0);
+      if (lane == 0) nacc += (unsigned long long)__popcll(bal);
+      if (mn) {
+        double s[6] = {(double)c11, (double)c22, (double)c12, (double)c33, (double)f13, (double)f23};
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off, 64);
+        if (lane == 0)
+          for (int q = 0; q < 6; ++q) red[q * (NX / 64) + (tid >> 6)] = s[q];
+      }
+      cx<T>* O = a.O;
+      const size_t o = (size_t)iz * plane + rowoff;
+      const size_t os = a.sstride;
+      const T* kxp = a.kxp;
+      auto emit = [&](int q, int k, cx<T> fA, cx<T> fB) {
+        const cx<T> ikx = {0, kxp[k]};
+        switch (q) {
+          case 0: O[0 * os + o + k] = cadd(cmul(ikx, fA), fB); break;          // G_u
+          case 1: O[1 * os + o + k] = fA;                                       // Fy_u
+                  O[2 * os + o + k] = cadd(cmul(ikx, fA), fB); break;          // G_v
+          case 2: O[4 * os + o + k] = cadd(cmul(ikx, fA), fB); break;          // G_w
+          default: O[3 * os + o + k] = fA;                                      // Fy_v
+                   O[5 * os + o + k] = fB; break;                               // Fy_w
+        }
+      };
+      anal_small<T, NX, 4>(buf, fa, fb, tw, tid, emit);
+      __syncthreads();                     // wave partials published; buf free for the next level
+      if (mn && tid < 6) {
+        double sq = 0;
+        for (int w = 0; w < NX / 64; ++w) sq += red[tid * (NX / 64) + w];
+        atomicAdd(&a.sred[(size_t)tid * a.slots + iz], sq / ((double)NX * (double)NY));
+      }
+    } else {
+      __syncthreads();
+    }
+    um = u0; u0 = up; up = upp;
+    vm = v0; v0 = vp; vp = vpp;
+    wm = w0; w0 = wp;
+    em = e0; e0 = ep;
+    ux0 = ux1; uy0 = uy1; vx0 = vx1; vy0 = vy1;
+    wxm = wx0; wym = wy0; wx0 = wxp; wy0 = wyp;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t0[q] = t1[q];
+    ok0 = ok1;
+    fuw = f13; fvw = f23;
+  }
+  if (lane == 0) {
+    atomicAdd(a.counts, nacc);
+    atomicAdd(a.counts + 1, 64ull * (unsigned long long)(z1 - z0));
+  }
+}
+
+// R += G + i ky F^y on band spectra, slots 1..nl; with zero_mean, the (0,0)
+// mode is left unchanged (the plane-mean removal of rw, comp1.f90:172).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sgs_add(cx<T>* R, const cx<T>* __restrict__ G, const cx<T>* __restrict__ Fy,
+                                                 const T* __restrict__ kyp, int mx, int ny, int nl, int zero_mean) {
+  const size_t plane = (size_t)mx * ny;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= plane * nl) return;
+  const size_t m = i % plane;
+  if (zero_mean && m == 0) return;
+  const size_t o = plane + i;
+  const int j = (int)(m / mx);
+  const T ky = kyp[j];
+  const cx<T> g = G[o], f = Fy[o], r = R[o];
+  R[o] = cx<T>{r.x + g.x - ky * f.y, r.y + g.y + ky * f.x};
 }
 
 // ---------------------------------------------------------------------------

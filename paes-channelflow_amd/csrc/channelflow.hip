@@ -42,8 +42,9 @@ thread_local std::string g_create_error;
 
 constexpr double PI = 3.14159265358979323846264338327950288;
 
-const char* kKernelNames[CF_NKERN] = {"k_wall",  "k_col_synth", "k_rows",        "k_col_anal",
-                                      "k_press", "k_erows",     "k_reduce_part", "(unused)"};
+const char* kKernelNames[CF_NKERN] = {"k_wall",     "k_col_synth",    "k_rows",  "k_col_anal",
+                                      "k_sgs_cols_in", "k_sgs_rows", "k_sgs_cols_out",
+                                      "k_press",    "k_erows",        "k_reduce_part"};
 
 }  // namespace
 
@@ -83,6 +84,11 @@ struct cf_ctx {
   cf::mlp::Net* closure_net = nullptr;
   unsigned long long* closure_counts = nullptr;   // accepted, evaluated cells
   bool closure_on = false;
+  int closure_grid = CF_CLOSURE_GRID_DEALIASED;
+  void* sgsS = nullptr;         // small-grid closure: 6 x-spectral input rows (band-sized each)
+  void* sgsO = nullptr;         // and its 6 output rows
+  double* sred = nullptr;       // its SGS means [6][slots]
+  bool have_sred = false;       // sred holds the last mnout stage's means
   // box-filter work buffers (allocated on first use)
   double* filt_in = nullptr;    // u, v, w, e physical, 4 * nz*ny*nx
   double* filt_out = nullptr;
@@ -217,6 +223,7 @@ struct EngineT : Engine {
     const double dsl = std::pow(dx * dy * dz, 1.0 / 3.0);                // les.F90:118
     c.alk = p.alk > 0 ? p.alk : dsl;
     c.dsl = dsl;
+    c.sgs_momentum = (ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL) ? 0 : 1;
     c.xkmax = dz * dz / ctx->dt / 5.0;                                  // les.F90:216
     c.ceps = 0.93 / dsl;                                                // les.F90:120
     c.emin = 1.0e-6;                                                    // les.F90:102
@@ -282,7 +289,7 @@ struct EngineT : Engine {
       a.c = c;
       bool learned = false;
       if constexpr ((3 * NX / 2) % 64 == 0) {
-        if (ctx->closure_on) {
+        if (ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_DEALIASED) {
           // longer chunks: each chunk re-evaluates the network at two extra levels
           a.c.zchunk = ctx->zchunk_mlp;
           const int nchm = (nz + ctx->zchunk_mlp - 1) / ctx->zchunk_mlp;
@@ -310,6 +317,19 @@ struct EngineT : Engine {
       const int ntiles = (MX / CS::TK) * nz * 4;
       const int grid = std::min(ntiles, 2 * ctx->ncu);
       hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(grid), dim3(CS::NT), 0, s, a);
+    }
+    mark();
+    bool small = false;
+    if constexpr (NX % 64 == 0) {
+      if (ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL) {
+        small = true;
+        int rc = sgs_small(ctx, c, sgs, mark);
+        if (rc != CF_OK) return rc;
+      }
+    }
+    if (!small) {
+      mark();
+      mark();
     }
     mark();
     {
@@ -366,7 +386,10 @@ struct EngineT : Engine {
       ctx->err = std::string("stage launch: ") + hipGetErrorString(e);
       return CF_E_HIP;
     }
-    if (sgs) ctx->have_sgs = true;
+    if (sgs) {
+      ctx->have_sgs = true;
+      ctx->have_sred = ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL;
+    }
     return CF_OK;
   }
 
@@ -392,6 +415,62 @@ struct EngineT : Engine {
                        ctx->stream, src, xs, tw, 1);
     hipLaunchKernelGGL((cf::k_rows_inv<Q, NX>), dim3(NY / 2, nz), dim3(MX), 0, ctx->stream, xs,
                        ctx->staging, tw, 1);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+
+  // The learned closure on the nnx x nny grid: y-inverse of u, v, w and their
+  // y-derivatives to x-spectral small rows, the row kernel, then the
+  // y-forward of its divergence rows added to the comp1 tendencies.
+  // mark() is called between the three sections (columns in, rows, columns out).
+  template <typename Mark>
+  int sgs_small(cf_ctx* ctx, const cf::Consts& c, bool sgs, Mark& mark) {
+    hipStream_t s = ctx->stream;
+    const int nz = ctx->nz;
+    const size_t S = ctx->slots;
+    const size_t fstride = S * NY * MX;
+    const size_t nband = (size_t)nz * NY * MX;
+    constexpr int TK = CS::TK;
+    C* in = cptr(ctx->sgsS);
+    C* out = cptr(ctx->sgsO);
+    void* src[3] = {ctx->U, ctx->V, ctx->W};
+    for (int f = 0; f < 3; ++f) {
+      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, true>), dim3(MX / TK, nz), dim3(256), 0, s,
+                         cptr(src[f]), in + f * fstride, cptr(ctx->twNX), 1);
+      hipLaunchKernelGGL((cf::k_band_deriv<T>), dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, s,
+                         cptr(src[f]), cptr(ctx->band_tmp), rptr(ctx->kxp), rptr(ctx->kyp), MX, NY, nz, 1);
+      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, true>), dim3(MX / TK, nz), dim3(256), 0, s,
+                         cptr(ctx->band_tmp), in + (3 + f) * fstride, cptr(ctx->twNX), 1);
+    }
+    if (sgs) HIPCHK(hipMemsetAsync(ctx->sred, 0, 6 * S * sizeof(double), s));
+    mark();
+    cf::SgsRowArgs<T> a{};
+    a.S = in;
+    a.sstride = fstride;
+    a.O = out;
+    a.e = rptr(ctx->e);
+    a.kxp = rptr(ctx->kxp);
+    a.twNX = cptr(ctx->twNX);
+    a.sred = ctx->sred;
+    a.slots = (int)S;
+    a.counts = ctx->closure_counts;
+    a.c = c;
+    a.c.zchunk = ctx->zchunk_mlp;
+    const int nch = (nz + ctx->zchunk_mlp - 1) / ctx->zchunk_mlp;
+    hipLaunchKernelGGL((cf::k_sgs_rows<T, NX>), dim3(NY, nch), dim3(NX), 0, s, a, ctx->closure_net);
+    mark();
+    void* R[3] = {ctx->Run, ctx->Rvn, ctx->Rwn};
+    for (int f = 0; f < 3; ++f) {
+      const int nl = f == 2 ? nz - 1 : nz;
+      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, false>), dim3(MX / TK, nl), dim3(256), 0, s,
+                         out + (2 * f) * fstride, cptr(ctx->band_tmp), cptr(ctx->twNX), 1);
+      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, false>), dim3(MX / TK, nl), dim3(256), 0, s,
+                         out + (2 * f + 1) * fstride, cptr(ctx->xs_tmp), cptr(ctx->twNX), 1);
+      const size_t nadd = (size_t)nl * NY * MX;
+      hipLaunchKernelGGL((cf::k_sgs_add<T>), dim3((unsigned)((nadd + 255) / 256)), dim3(256), 0, s,
+                         cptr(R[f]), cptr(ctx->band_tmp), cptr(ctx->xs_tmp), rptr(ctx->kyp), MX, NY, nl,
+                         f == 2 ? 1 : 0);
+    }
     HIPCHK(hipGetLastError());
     return CF_OK;
   }
@@ -426,6 +505,7 @@ struct EngineT : Engine {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     ctx->have_sgs = false;
+    ctx->have_sred = false;
     return CF_OK;
   }
 
@@ -754,6 +834,12 @@ int cf_compmn(cf_ctx* ctx, double* meanbuf) {
     // SGS means uusb vvsb uvsb wwsb uwsb vwsb (allocate.f90:289-294)
     for (int r = 0; r < 6; ++r) mb[13 + r] = ctx->have_sgs ? red[(size_t)(r + 1) * ctx->slots + i + 1] : 0.0;
   }
+  if (ctx->have_sgs && ctx->have_sred) {
+    std::vector<double> sr(6 * (size_t)ctx->slots);
+    HIPCHK(hipMemcpy(sr.data(), ctx->sred, sr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nz; ++i)
+      for (int r = 0; r < 6; ++r) meanbuf[19 * (size_t)i + 13 + r] += sr[(size_t)r * ctx->slots + i + 1];
+  }
   return CF_OK;
 }
 
@@ -825,7 +911,7 @@ int cf_time_steps(cf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms) 
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (kernel_ms) {
-    constexpr int NE = 8;
+    constexpr int NE = CF_NKERN + 1;
     for (int k = 0; k < CF_NKERN; ++k) kernel_ms[k] = 0;
     std::vector<hipEvent_t> ev(NE);
     for (auto& x : ev) HIPCHK(hipEventCreate(&x));
@@ -1286,6 +1372,29 @@ int cf_set_closure(cf_ctx* ctx, const cf_mlp* m) {
   HIPCHK(hipMemsetAsync(ctx->closure_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->closure_on = true;
+  return CF_OK;
+}
+
+int cf_set_closure_grid(cf_ctx* ctx, int32_t grid) {
+  if (!ctx) return CF_E_ARG;
+  if (grid != CF_CLOSURE_GRID_DEALIASED && grid != CF_CLOSURE_GRID_SMALL) {
+    ctx->err = "closure grid must be CF_CLOSURE_GRID_DEALIASED or CF_CLOSURE_GRID_SMALL";
+    return CF_E_ARG;
+  }
+  HIPCHK(hipSetDevice(ctx->p.device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (grid == CF_CLOSURE_GRID_SMALL && !ctx->sgsS) {
+    const size_t elem = ctx->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    const size_t band = (size_t)ctx->slots * ctx->ny * ctx->mx * elem;
+    void* p = nullptr;
+    int rc;
+    if ((rc = dalloc(ctx, (char**)&p, 6 * band)) != CF_OK) return rc;
+    ctx->sgsS = p;
+    if ((rc = dalloc(ctx, (char**)&p, 6 * band)) != CF_OK) return rc;
+    ctx->sgsO = p;
+    if ((rc = dalloc(ctx, &ctx->sred, 6 * (size_t)ctx->slots)) != CF_OK) return rc;
+  }
+  ctx->closure_grid = grid;
   return CF_OK;
 }
 

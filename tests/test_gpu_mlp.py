@@ -15,6 +15,8 @@ through the C-ABI.
   ~3e-4 relative in one stage; GPU and oracle fields agree within 1e-6
   relative L2, and the same cells accept within 0.1 %.  With a network that
   every cell rejects, fields agree within 1e-10 of the TKE stage.
+* The same with the network on the nnx x nny grid, against
+  sgs_mlp.LearnedClosureSmall, plus the stage-1 SGS mean profiles.
 """
 import numpy as np
 import pytest
@@ -177,6 +179,60 @@ def test_learned_stage_matches_oracle(kind, nz, stages):
             assert acc == 0
         else:
             assert 0.05 * tot < acc < 0.95 * tot
+    g.close()
+    net.close()
+
+
+@pytest.mark.parametrize("kind,nz,stages", [("reject", 8, (1, 2, 3)), ("mixed", 8, (1, 2, 3)),
+                                            ("mixed", 64, (1,))])
+def test_small_grid_closure_matches_oracle(kind, nz, stages):
+    """The closure evaluated on the nnx x nny grid (CF_CLOSURE_GRID_SMALL)
+    against sgs_mlp.LearnedClosureSmall, with the stage-1 SGS means."""
+    import channelflow as C
+    pr0 = O.Params(128, 128, nz, dt=0.002)
+    u, v, w, e = O.initial_fields(pr0)
+    flat, norm = _closure_net(kind)
+    onet = M.Network(flat, *[np.asarray(a, float) for a in norm], bf16_mode=True)
+    cl = M.LearnedClosureSmall(onet)
+    pr = O.Params(128, 128, nz, dt=0.002, closure=cl)
+    st = O.set_state(pr, u, v, w, e)
+    g = _les(pr, C.CF_PREC_FP64)
+    g.set_state(u, v, w, e)
+    net = make_net(flat, norm)
+    g.set_closure(net)
+    g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL)
+    g.closure_counts()
+    for istage in stages:
+        mn = istage == 1
+        O.stage(st, pr, istage, mnout=mn)
+        g.stage(istage, mnout=mn)
+        acc, tot = g.closure_counts()
+        ref_acc = int(cl.last_accept.sum())
+        gs = g.get_state()
+        errs = {k: rel(gs[k], O.interior(st, pr, k)) for k in "uvwe"}
+        print("small", kind, istage, acc, ref_acc, tot, cl.last_accept.size, errs, flush=True)
+        assert tot == cl.last_accept.size
+        assert abs(acc - ref_acc) <= max(2, 1e-3 * tot)
+        tol = 1e-10 if kind == "reject" else 1e-6
+        for k in "uvwe":
+            assert errs[k] < tol, (istage, k, errs)
+        if kind == "reject":
+            assert acc == 0
+        else:
+            assert 0.05 * tot < acc < 0.95 * tot
+        if mn:
+            # Each profile against its own largest value, floored at 1e-6 of the
+            # largest SGS mean (the wall stress, ~1.6e-2): with a rejecting
+            # network uusb is rounding noise (~1e-22, K is uniform per level at
+            # this state).  The network's stresses are BF16-rounded (5e-3
+            # relative per point), hence 1e-2 with the mixed network.
+            mg, mo = g.compmn(), O.compmn(st, pr)
+            rows = [O.MEAN_NAMES.index(n) for n in ("uusb", "vvsb", "uvsb", "wwsb", "uwsb", "vwsb")]
+            scale = max(float(np.max(np.abs(mo[i]))) for i in rows)
+            for i in rows:
+                d = float(np.max(np.abs(mg[i] - mo[i]))) / max(float(np.max(np.abs(mo[i]))), 1e-6 * scale)
+                print("  mean", O.MEAN_NAMES[i], f"{d:.2e}", flush=True)
+                assert d < (1e-9 if kind == "reject" else 1e-2), (O.MEAN_NAMES[i], d)
     g.close()
     net.close()
 

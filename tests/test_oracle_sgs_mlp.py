@@ -235,3 +235,70 @@ def test_learned_stage_keeps_projection_invariants():
     assert np.abs(div).max() < 1e-12 * scale
     wm = O.interior(st, pr, "w")[:-1].mean(axis=(1, 2))
     assert np.abs(wm).max() < 1e-15
+
+
+# ----------------------------------------------------------------------------
+# small-grid mode
+# ----------------------------------------------------------------------------
+def test_small_grid_all_fallback_equals_tke_on_the_small_grid():
+    """With every cell rejected, the small-grid mode is the TKE closure with
+    its stresses evaluated on the small grid.  Against the 3/2-grid TKE stage
+    that is a different discretisation: close, not equal."""
+    pr0 = O.Params(16, 16, 8, dt=0.01)
+    u, v, w, e = O.initial_fields(pr0)
+    flat = M.init_params(1)
+    L = M.unpack(flat)
+    L[3][0][:] = 0.0
+    L[3][1][:] = [-1.0, 1, 1, 0, 0, 0]
+    net = M.Network(M.pack(L), np.zeros(11), np.ones(11), np.zeros(6), np.ones(6))
+    cl = M.LearnedClosureSmall(net)
+    pr = O.Params(16, 16, 8, dt=0.01, closure=cl)
+    a = O.set_state(pr0, u, v, w, e)
+    b = O.set_state(pr, u, v, w, e)
+    O.stage(a, pr0, 1)
+    O.stage(b, pr, 1)
+    assert not cl.last_accept.any()
+    for k in "uvw":
+        r = np.linalg.norm(O.interior(b, pr, k) - O.interior(a, pr0, k)) / np.linalg.norm(O.interior(a, pr0, k))
+        assert 0 < r < 1e-3, (k, r)
+
+
+def test_small_grid_stresses_by_hand_at_one_point():
+    """The TKE fallback stresses of the small-grid mode at one point, from the
+    formulas of compsgs1.f90:144-153 written out directly."""
+    pr = O.Params(16, 16, 8, dt=0.01)
+    u, v, w, e = O.initial_fields(pr)
+    st = O.set_state(pr, u, v, w, e)
+    O.lower(st, pr); O.upper(st, pr)
+    flat = M.init_params(1)
+    L = M.unpack(flat)
+    L[3][0][:] = 0.0
+    L[3][1][:] = [-1.0, 1, 1, 0, 0, 0]
+    net = M.Network(M.pack(L), np.zeros(11), np.ones(11), np.zeros(6), np.ones(6))
+    (c11, c22, c12, c33), (f13, f23), ok = M.small_grid_stresses(st, pr, net)
+    k, j, i = 3, 5, 7                                   # level 3, row 5, column 7
+    lv = lambda a, L: O.lev(a, L)                       # noqa: E731
+    K = lambda f: min(0.1 * pr.alk * math.sqrt(lv(st.e, f)[j, i]), pr.xkmax)   # noqa: E731
+    ux = O.xderiv(lv(st.u, k), pr.xl)[j, i]
+    wz = (lv(st.w, k) - lv(st.w, k - 1))[j, i] / pr.dz
+    assert abs(c11[k - 1, j, i] - (-(K(k - 1) + K(k)) * ux)) < 1e-14
+    assert abs(c33[k - 1, j, i] - (-(K(k - 1) + K(k)) * wz)) < 1e-14
+    uzf = (lv(st.u, k + 1) - lv(st.u, k))[j, i] / pr.dz
+    wxf = O.xderiv(lv(st.w, k), pr.xl)[j, i]
+    assert abs(f13[k - 1, j, i] - (-K(k) * (uzf + wxf))) < 1e-14
+
+
+def test_small_grid_stage_keeps_projection_invariants():
+    pr0 = O.Params(16, 16, 8, dt=0.01)
+    u, v, w, e = O.initial_fields(pr0)
+    net = M.Network(M.init_params(5), np.zeros(11), np.full(11, 0.05),
+                    np.array([1e-3, 1e-3, 1e-3, 0, 0, 0]), np.full(6, 1e-4))
+    cl = M.LearnedClosureSmall(net)
+    pr = O.Params(16, 16, 8, dt=0.01, closure=cl)
+    st = O.set_state(pr, u, v, w, e)
+    O.step(st, pr, mnout=True)
+    assert 0 < cl.last_accept.mean() < 1
+    div = O.divergence(st, pr)
+    scale = np.abs(O.xderiv(O.interior(st, pr, "u"), pr.xl)).max()
+    assert np.abs(div).max() < 1e-12 * scale
+    assert np.abs(O.interior(st, pr, "w")[:-1].mean(axis=(1, 2))).max() < 1e-15
