@@ -56,13 +56,13 @@ def kernel_bytes(name, n, s, istage=2):
         "k_press": 15 * n * s + 8 * n,
         # x-spectral re, e, re in; e, re, x-spectral e out
         "k_erows": (6 if istage > 1 else 5) * n * s,
-        # small-grid closure: U, V, W -> u, v, w, uy, vy, wy small rows (6 column
-        # transforms, 3 i ky products), each a band in and a band out
-        "k_sgs_cols_in": 18 * n * s,
+        # small-grid closure: U, V, W band spectra in; u, v, w, uy, vy, wy small
+        # x-spectral rows out
+        "k_sgs_cols_in": 9 * n * s,
         # 6 small rows + physical e in; G and F^y rows of u, v, w out
         "k_sgs_rows": 13 * n * s,
-        # 6 column transforms (band in, band out) and R += G + i ky F^y (3 in, 1 out)
-        "k_sgs_cols_out": 24 * n * s,
+        # G and F^y rows of u, v, w in; ru, rv, rw read and written
+        "k_sgs_cols_out": 12 * n * s,
     }.get(name)
 
 BASIS = {
@@ -97,8 +97,10 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain path")
     ap.add_argument("--no-tke32", action="store_true", help="skip the FP32 TKE-closure path")
     ap.add_argument("--weights", default=WEIGHTS, help="learned closure network file")
-    ap.add_argument("--closure-grid", choices=("dealiased", "small"), default="dealiased",
-                    help="evaluate the network on the 3/2 grid or on the nnx x nny grid")
+    ap.add_argument("--closure-grid", choices=("dealiased", "small"), default="small",
+                    help="evaluate the network on the nnx x nny grid (default) or on the 3/2 grid")
+    ap.add_argument("--no-other-grid", action="store_true",
+                    help="skip the learned path with the network on the other grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=0,
                     help="levels of the CPU-baseline sample (default: the full nz)")
@@ -107,7 +109,7 @@ def parse():
     return ap.parse_args()
 
 
-def run_gpu(args, precision_name, learned=False):
+def run_gpu(args, precision_name, learned=False, grid=None):
     import numpy as np
     import channelflow as C
     nx, nz = args.nx, args.nz
@@ -124,7 +126,7 @@ def run_gpu(args, precision_name, learned=False):
     if learned:
         net = C.SgsNet(0, args.weights)
         g.set_closure(net)
-        g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL if args.closure_grid == "small"
+        g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL if (grid or args.closure_grid) == "small"
                            else C.CF_CLOSURE_GRID_DEALIASED)
     if args.warmup > 0:
         g.step(args.warmup)
@@ -199,6 +201,11 @@ def main():
     print(f"[bench] fp32 + learned closure {args.nx}x{args.nx}x{args.nz}", file=sys.stderr, flush=True)
     r32 = run_gpu(args, "fp32", learned=True)
     value = pts * args.steps / (r32["ms"] * 1e-3)
+    other = "dealiased" if args.closure_grid == "small" else "small"
+    ro = None
+    if not args.no_other_grid:
+        print(f"[bench] fp32 + learned closure, network on the {other} grid", file=sys.stderr, flush=True)
+        ro = run_gpu(args, "fp32", learned=True, grid=other)
     print("[bench] fp64 plain path", file=sys.stderr, flush=True)
     r64 = None if args.no_fp64 else run_gpu(args, "fp64")
     print("[bench] fp32 TKE path", file=sys.stderr, flush=True)
@@ -230,7 +237,8 @@ def main():
         "data": "synthetic: seeded log-law + 0.1 u_tau uniform noise, e = 0.01 u_tau^2",
         "config": {
             "workload": f"channel {args.nx}x{args.nx}x{args.nz}, 2pi x pi x 2, Re_tau={args.re_tau:g} "
-                        f"(z0={r32['z0']:.4e}), learned SGS closure (MLP 11-64-64-64-6, BF16 MFMA) "
+                        f"(z0={r32['z0']:.4e}), learned SGS closure (MLP 11-64-64-64-6, BF16 MFMA, on the "
+                        f"{'nnx x nny' if args.closure_grid == 'small' else '3/2'} grid) "
                         f"with TKE fallback (imodel 3), Schumann wall (isurface 1), "
                         f"3/2 dealiasing, dt={r32['dt']:.3e}",
             "grid": [args.nx, args.nx, args.nz],
@@ -267,6 +275,11 @@ def main():
         vt = pts * args.steps / (rt["ms"] * 1e-3)
         line["fp32_tke"] = {"value": vt, "ms_per_step": rt["ms"] / args.steps,
                             "kernel_ms_per_stage": rt["per_kernel"], "finite": rt["finite"]}
+    if ro is not None:
+        line["learned_other_grid"] = {
+            "closure_grid": other, "value": pts * args.steps / (ro["ms"] * 1e-3),
+            "ms_per_step": ro["ms"] / args.steps, "kernel_ms_per_stage": ro["per_kernel"],
+            "closure_cells_accepted": ro["accepted"][0] / max(ro["accepted"][1], 1), "finite": ro["finite"]}
     if r64 is not None:
         v64 = pts * args.steps / (r64["ms"] * 1e-3)
         line["fp64_plain"] = {

@@ -922,7 +922,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp
 //   v: Fx = -t12, Fy = -t22, H = -(t23(iz) - t23(iz-1))/dz;
 //   w (face iz): Fx = -t13(iz), Fy = -t23(iz), H = -(t33(iz+1) - t33(iz))/dz.
 // The wall faces 0 and nnz carry no stress here: the 3/2-grid sweep keeps
-// the wall model (k_rows with sgs_momentum == 0).  k_sgs_add then adds
+// the wall model (k_rows with sgs_momentum == 0).  k_sgs_cols_out then adds
 // G + i ky F^y, y-transformed, to the comp1 tendencies.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -990,22 +990,98 @@ __device__ __forceinline__ void synth_small(cx<T>* buf, const cx<T>* const* sa, 
   }
 }
 
+// The NP forward transforms go TPWA to a wave (NP <= NW * TPWA): with four
+// pairs and four waves, one each.
+// The half spectra this lane fills into the synthesis transforms
+// q = w*TPW + qq (modes k = lane + 64 kk), loaded a level ahead so that their
+// memory latency overlaps the network.
+template <typename T, int NX>
+struct SmallPre {
+  static constexpr int TPW = SmallRowFFT<T, NX>::TPW, KPL = (NX / 2 + 63) / 64;
+  cx<T> a[TPW][KPL], b[TPW][KPL];
+};
+
+template <typename T, int NX, int NP>
+__device__ __forceinline__ void small_prefetch(SmallPre<T, NX>& P, const cx<T>* const* sa,
+                                               const cx<T>* const* sb, int tid) {
+  using F = SmallRowFFT<T, NX>;
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int qq = 0; qq < F::TPW; ++qq) {
+    const int q = w * F::TPW + qq;
+    const cx<T>* pa = q < NP ? sa[q] : nullptr;
+    const cx<T>* pb = q < NP ? sb[q] : nullptr;
+#pragma unroll
+    for (int kk = 0; kk < SmallPre<T, NX>::KPL; ++kk) {
+      const int k = lane + 64 * kk;
+      P.a[qq][kk] = (pa && k < F::MX) ? pa[k] : cx<T>{0, 0};
+      P.b[qq][kk] = (pb && k < F::MX) ? pb[k] : cx<T>{0, 0};
+    }
+  }
+}
+
+// synth_small from prefetched half spectra.
+template <typename T, int NX, int NP>
+__device__ __forceinline__ void synth_small_pre(cx<T>* buf, const SmallPre<T, NX>& P, const bool* da,
+                                                const bool* db, T* ra, T* rb, const T* __restrict__ kxp,
+                                                const cx<T>* tw, int tid) {
+  using F = SmallRowFFT<T, NX>;
+  constexpr int L = NX, MX = F::MX;
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int qq = 0; qq < F::TPW; ++qq) {
+    const int q = w * F::TPW + qq;
+    if (q < NP) {
+      cx<T>* z = buf + q * padlen(NX);
+      const bool dqa = da[q], dqb = db[q];
+#pragma unroll
+      for (int kk = 0; kk < SmallPre<T, NX>::KPL; ++kk) {
+        const int k = lane + 64 * kk;
+        if (k < MX) {
+          cx<T> av = P.a[qq][kk], bv = P.b[qq][kk];
+          const T kx = kxp[k];
+          if (dqa) av = cx<T>{-kx * av.y, kx * av.x};
+          if (dqb) bv = cx<T>{-kx * bv.y, kx * bv.x};
+          if (k == 0) {
+            z[0] = cx<T>{av.x, bv.x};
+          } else {
+            z[pidx(k)] = cx<T>{av.x - bv.y, av.y + bv.x};
+            z[pidx(L - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
+          }
+        }
+      }
+      if (lane == 0) z[pidx(MX)] = cx<T>{0, 0};
+    }
+  }
+  wave_sync();
+  if (w * F::TPW < NP) fft_run<T, NX, true, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NX), tw, lane);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const cx<T> v = buf[q * padlen(NX) + pidx(tid)];
+    ra[q] = v.x;
+    rb[q] = v.y;
+  }
+}
+
 template <typename T, int NX, int NP, typename Emit>
 __device__ __forceinline__ void anal_small(cx<T>* buf, const T* xa, const T* xb, const cx<T>* tw, int tid,
                                            Emit emit) {
   using F = SmallRowFFT<T, NX>;
   constexpr int MX = F::MX;
+  constexpr int TPWA = (NP + F::NW - 1) / F::NW;
+  static_assert(F::NW * TPWA <= 6 || NP <= F::NW, "transforms stay inside the 6-slot buffer");
   const T s = T(0.5) / T(NX);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NP; ++q) buf[q * padlen(NX) + pidx(tid)] = cx<T>{xa[q], xb[q]};
   __syncthreads();
   const int w = tid >> 6, lane = tid & 63;
-  if (w * F::TPW < NP) {
-    fft_run<T, NX, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NX), tw, lane);
+  if (w * TPWA < NP) {
+    fft_run<T, NX, false, TPWA, 64, 0, true>(buf + w * TPWA * padlen(NX), tw, lane);
 #pragma unroll
-    for (int qq = 0; qq < F::TPW; ++qq) {
-      const int q = w * F::TPW + qq;
+    for (int qq = 0; qq < TPWA; ++qq) {
+      const int q = w * TPWA + qq;
       if (q < NP)
         for (int k = lane; k < MX; k += 64) {
           const cx<T> zk = buf[q * padlen(NX) + pidx(k)];
@@ -1113,26 +1189,38 @@ k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
     ok0 = closure(zs, g, t0);
   }
 
+  // the synthesis sources of iteration iz: u, v at level iz+2; ux, vx, uy, vy
+  // at iz+1; w, wx, wy at face iz+1
+  auto prefetch = [&](SmallPre<T, NX>& P, int iz) {
+    const cx<T>* sa[5];
+    const cx<T>* sb[5];
+    sa[0] = lvl(SU, iz + 2);  sb[0] = lvl(SV, iz + 2);
+    sa[1] = lvl(SU, iz + 1);  sb[1] = lvl(SV, iz + 1);      // ux, vx
+    sa[2] = lvl(SUY, iz + 1); sb[2] = lvl(SVY, iz + 1);     // uy, vy
+    sa[3] = face(SW, iz + 1); sb[3] = face(SW, iz + 1);     // w, wx
+    sa[4] = face(SWY, iz + 1); sb[4] = nullptr;             // wy
+    small_prefetch<T, NX, 5>(P, sa, sb, tid);
+  };
+  SmallPre<T, NX> P;
+  prefetch(P, zs);
+  T epre = e_at(zs + 1);
   for (int iz = zs; iz < z1; ++iz) {
     {
       T ra[5], rb[5];
-      const cx<T>* sa[5];
-      const cx<T>* sb[5];
       const bool da[5] = {false, true, false, false, false};
       const bool db[5] = {false, true, false, true, false};
-      sa[0] = lvl(SU, iz + 2);  sb[0] = lvl(SV, iz + 2);
-      sa[1] = lvl(SU, iz + 1);  sb[1] = lvl(SV, iz + 1);      // ux, vx
-      sa[2] = lvl(SUY, iz + 1); sb[2] = lvl(SVY, iz + 1);     // uy, vy
-      sa[3] = face(SW, iz + 1); sb[3] = face(SW, iz + 1);     // w, wx
-      sa[4] = face(SWY, iz + 1); sb[4] = nullptr;             // wy
-      synth_small<T, NX, 5>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      synth_small_pre<T, NX, 5>(buf, P, da, db, ra, rb, a.kxp, tw, tid);
       upp = ra[0]; vpp = rb[0];
       ux1 = ra[1]; vx1 = rb[1];
       uy1 = ra[2]; vy1 = rb[2];
       wp = ra[3]; wxp = rb[3];
       wyp = ra[4];
     }
-    ep = e_at(iz + 1);
+    ep = epre;
+    if (iz + 1 < z1) {                     // next level's loads fly during the network
+      prefetch(P, iz + 1);
+      epre = e_at(iz + 2);
+    }
     float t1[6];
     bool ok1;
     {
@@ -1227,21 +1315,97 @@ k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
   }
 }
 
-// R += G + i ky F^y on band spectra, slots 1..nl; with zero_mean, the (0,0)
-// mode is left unchanged (the plane-mean removal of rw, comp1.f90:172).
+// Small-grid closure, columns in: U, V, W band spectra -> u, v, w and uy, vy,
+// wy as x-spectral small rows (the y-inverse of a tile of TK columns and of
+// its i ky multiple in one batch).  grid (MX/TK, nz, 3 fields)
 template <typename T>
-__global__ void __launch_bounds__(256) k_sgs_add(cx<T>* R, const cx<T>* __restrict__ G, const cx<T>* __restrict__ Fy,
-                                                 const T* __restrict__ kyp, int mx, int ny, int nl, int zero_mean) {
-  const size_t plane = (size_t)mx * ny;
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= plane * nl) return;
-  const size_t m = i % plane;
-  if (zero_mean && m == 0) return;
-  const size_t o = plane + i;
-  const int j = (int)(m / mx);
-  const T ky = kyp[j];
-  const cx<T> g = G[o], f = Fy[o], r = R[o];
-  R[o] = cx<T>{r.x + g.x - ky * f.y, r.y + g.y + ky * f.x};
+struct SgsColsInArgs {
+  const cx<T>* U[3];      // U, V, W band spectra
+  cx<T>* S;               // outputs u, v, w, uy, vy, wy [6][slot][NY][MX]
+  size_t sstride;
+  const T* kyp;
+  const cx<T>* twNY;
+  int nz;
+};
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(256) k_sgs_cols_in(SgsColsInArgs<T> a) {
+  using Sh = ColShape<T, NX, NY>;
+  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 256, PL = padlen(NY);
+  const int f = blockIdx.z, z = blockIdx.y + 1;
+  if (f == 2 && z >= a.nz) return;                 // w lives on faces 1..nz-1
+  const int k0 = blockIdx.x * TK, tid = threadIdx.x;
+  __shared__ cx<T> buf[2 * TK * PL];
+  __shared__ cx<T> tw[NY];
+  load_twiddles(tw, a.twNY, NY, tid, NT);
+  const cx<T>* src = a.U[f] + (size_t)z * NY * MX + k0;
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, j = i / TK;
+    cx<T> v = src[(size_t)j * MX + c];
+    if (j == NY / 2) v = cx<T>{0, 0};
+    const T k = a.kyp[j];
+    buf[c * PL + pidx(j)] = v;
+    buf[(TK + c) * PL + pidx(j)] = cx<T>{-k * v.y, k * v.x};
+  }
+  __syncthreads();
+  fft_waves<T, NY, true, 2 * TK, NT / 64>(buf, tw, tid);
+  __syncthreads();
+  const size_t off = (size_t)z * NY * MX + k0;
+  cx<T>* o0 = a.S + (size_t)f * a.sstride + off;
+  cx<T>* o1 = a.S + (size_t)(3 + f) * a.sstride + off;
+  for (int i = tid; i < 2 * TK * NY; i += NT) {
+    const int c = i % TK, r = i / TK;
+    const int y = r % NY, d = r / NY;
+    (d ? o1 : o0)[(size_t)y * MX + c] = buf[(d * TK + c) * PL + pidx(y)];
+  }
+}
+
+// Small-grid closure, columns out: the y-forward of the G and F^y rows of one
+// component, then R += G + i ky F^y on the band (Nyquist row untouched; for w
+// the (0,0) mode too, the plane-mean removal of comp1.f90:172).
+// grid (MX/TK, nz, 3 components)
+template <typename T>
+struct SgsColsOutArgs {
+  const cx<T>* O;         // G_u, Fy_u, G_v, Fy_v, G_w, Fy_w [6][slot][NY][MX]
+  size_t ostride;
+  cx<T>* R[3];            // ru, rv, rw band spectra
+  const T* kyp;
+  const cx<T>* twNY;
+  int nz;
+};
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(256) k_sgs_cols_out(SgsColsOutArgs<T> a) {
+  using Sh = ColShape<T, NX, NY>;
+  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 256, PL = padlen(NY);
+  const int cmp = blockIdx.z, z = blockIdx.y + 1;
+  if (cmp == 2 && z >= a.nz) return;               // rw(nnz) = 0
+  const int k0 = blockIdx.x * TK, tid = threadIdx.x;
+  __shared__ cx<T> buf[2 * TK * PL];
+  __shared__ cx<T> tw[NY];
+  load_twiddles(tw, a.twNY, NY, tid, NT);
+  const size_t off = (size_t)z * NY * MX + k0;
+  const cx<T>* g = a.O + (size_t)(2 * cmp) * a.ostride + off;
+  const cx<T>* fy = g + a.ostride;
+  for (int i = tid; i < 2 * TK * NY; i += NT) {
+    const int c = i % TK, r = i / TK;
+    const int y = r % NY, d = r / NY;
+    buf[(d * TK + c) * PL + pidx(y)] = (d ? fy : g)[(size_t)y * MX + c];
+  }
+  __syncthreads();
+  fft_waves<T, NY, false, 2 * TK, NT / 64>(buf, tw, tid);
+  __syncthreads();
+  const T s = T(1) / T(NY);
+  cx<T>* R = a.R[cmp] + off;
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, j = i / TK;
+    if (j == NY / 2 || (cmp == 2 && j == 0 && k0 + c == 0)) continue;
+    const cx<T> gv = buf[c * PL + pidx(j)], fv = buf[(TK + c) * PL + pidx(j)];
+    const T k = a.kyp[j];
+    cx<T>& r = R[(size_t)j * MX + c];
+    const cx<T> rv = r;
+    r = cx<T>{rv.x + (gv.x - k * fv.y) * s, rv.y + (gv.y + k * fv.x) * s};
+  }
 }
 
 // ---------------------------------------------------------------------------

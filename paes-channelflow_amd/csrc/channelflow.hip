@@ -429,18 +429,20 @@ struct EngineT : Engine {
     const int nz = ctx->nz;
     const size_t S = ctx->slots;
     const size_t fstride = S * NY * MX;
-    const size_t nband = (size_t)nz * NY * MX;
     constexpr int TK = CS::TK;
     C* in = cptr(ctx->sgsS);
     C* out = cptr(ctx->sgsO);
-    void* src[3] = {ctx->U, ctx->V, ctx->W};
-    for (int f = 0; f < 3; ++f) {
-      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, true>), dim3(MX / TK, nz), dim3(256), 0, s,
-                         cptr(src[f]), in + f * fstride, cptr(ctx->twNX), 1);
-      hipLaunchKernelGGL((cf::k_band_deriv<T>), dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, s,
-                         cptr(src[f]), cptr(ctx->band_tmp), rptr(ctx->kxp), rptr(ctx->kyp), MX, NY, nz, 1);
-      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, true>), dim3(MX / TK, nz), dim3(256), 0, s,
-                         cptr(ctx->band_tmp), in + (3 + f) * fstride, cptr(ctx->twNX), 1);
+    {
+      cf::SgsColsInArgs<T> a{};
+      a.U[0] = cptr(ctx->U);
+      a.U[1] = cptr(ctx->V);
+      a.U[2] = cptr(ctx->W);
+      a.S = in;
+      a.sstride = fstride;
+      a.kyp = rptr(ctx->kyp);
+      a.twNY = cptr(ctx->twNX);
+      a.nz = nz;
+      hipLaunchKernelGGL((cf::k_sgs_cols_in<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(256), 0, s, a);
     }
     if (sgs) HIPCHK(hipMemsetAsync(ctx->sred, 0, 6 * S * sizeof(double), s));
     mark();
@@ -459,17 +461,17 @@ struct EngineT : Engine {
     const int nch = (nz + ctx->zchunk_mlp - 1) / ctx->zchunk_mlp;
     hipLaunchKernelGGL((cf::k_sgs_rows<T, NX>), dim3(NY, nch), dim3(NX), 0, s, a, ctx->closure_net);
     mark();
-    void* R[3] = {ctx->Run, ctx->Rvn, ctx->Rwn};
-    for (int f = 0; f < 3; ++f) {
-      const int nl = f == 2 ? nz - 1 : nz;
-      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, false>), dim3(MX / TK, nl), dim3(256), 0, s,
-                         out + (2 * f) * fstride, cptr(ctx->band_tmp), cptr(ctx->twNX), 1);
-      hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, false>), dim3(MX / TK, nl), dim3(256), 0, s,
-                         out + (2 * f + 1) * fstride, cptr(ctx->xs_tmp), cptr(ctx->twNX), 1);
-      const size_t nadd = (size_t)nl * NY * MX;
-      hipLaunchKernelGGL((cf::k_sgs_add<T>), dim3((unsigned)((nadd + 255) / 256)), dim3(256), 0, s,
-                         cptr(R[f]), cptr(ctx->band_tmp), cptr(ctx->xs_tmp), rptr(ctx->kyp), MX, NY, nl,
-                         f == 2 ? 1 : 0);
+    {
+      cf::SgsColsOutArgs<T> b{};
+      b.O = out;
+      b.ostride = fstride;
+      b.R[0] = cptr(ctx->Run);
+      b.R[1] = cptr(ctx->Rvn);
+      b.R[2] = cptr(ctx->Rwn);
+      b.kyp = rptr(ctx->kyp);
+      b.twNY = cptr(ctx->twNX);
+      b.nz = nz;
+      hipLaunchKernelGGL((cf::k_sgs_cols_out<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(256), 0, s, b);
     }
     HIPCHK(hipGetLastError());
     return CF_OK;

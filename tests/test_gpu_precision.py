@@ -17,6 +17,7 @@
   a-priori mean SGS dissipation is 0.010 against the TKE's 0.024 and the
   filtered fine run's 0.012).  The seeded noise therefore evolves differently
   near the walls, and the fluctuation fields decorrelate within 200 steps.
+* The same with the network on the nnx x nny grid (CF_CLOSURE_GRID_SMALL).
 * 32^3 for 100 steps (the CPU-reference case): FP64 fields within 1e-9 of
   the oracle.
 """
@@ -34,7 +35,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def _run(nx, nz, steps, precision, learned=False):
+def _run(nx, nz, steps, precision, learned=False, small_grid=False):
     import channelflow as C
     z0 = C.z0_of_retau(395.0)
     dt = 0.3 / (3.2 * (3 * nx / 2) / (2 * math.pi) + 0.5 / (2.0 / nz))
@@ -47,6 +48,8 @@ def _run(nx, nz, steps, precision, learned=False):
                          "paes-channelflow_amd", "weights", "sgs_mlp.cfmlp")
         net = C.SgsNet(0, w)
         g.set_closure(net)
+        if small_grid:
+            g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL)
     g.step(steps - 1)
     for s in (1, 2, 3):
         g.stage(s, mnout=True)
@@ -100,6 +103,16 @@ def test_learned_path_against_fp64_tke_256():
     e = _report("fp32+learned vs fp64 TKE after 200 steps:", sl, s64, ml, m64, 128)
     assert e["uxym"] < 1e-2, e                     # mean profile within 1 %
     assert abs(e["englez_ratio"] - 1.0) < 5e-2, e  # total resolved TKE within 5 %
+
+
+def test_learned_small_grid_against_fp64_tke_256():
+    """The network on the nnx x nny grid (CF_CLOSURE_GRID_SMALL), same bounds."""
+    import channelflow as C
+    s64, m64 = _run(256, 128, 200, C.CF_PREC_FP64)
+    sl, ml = _run(256, 128, 200, C.CF_PREC_FP32, learned=True, small_grid=True)
+    e = _report("fp32+learned (small grid) vs fp64 TKE after 200 steps:", sl, s64, ml, m64, 128)
+    assert e["uxym"] < 1e-2, e
+    assert abs(e["englez_ratio"] - 1.0) < 5e-2, e
 
 
 def test_fp64_100_steps_32cubed_matches_oracle():
