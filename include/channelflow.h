@@ -53,8 +53,12 @@ typedef struct cf_params {
   double xl, yl, zl;       /* domain lengths                                    */
   double z0;               /* roughness length (wall model)                     */
   double dt;               /* fixed time step (xkmax = dz^2/dt/5, les.F90:216)   */
-  int32_t imodel;          /* SGS length scale; 3 supported (complen.f90:88-89) */
-  int32_t isurface;        /* wall model; 1 supported (lowerbc.f90:128-130)     */
+  int32_t imodel;          /* SGS diffusivity: 3 TKE (Moeng 84), 4 Smagorinsky   */
+                           /* (complen.f90:19-24); the learned closure needs 3.  */
+  int32_t isurface;        /* wall model 1..6 (lowerbc.f90:126-161): 1 Schumann, */
+                           /* 2 Moeng, 3 Wei, 4 Moeng unlinearised, 5 Wei with   */
+                           /* Moeng's fluctuation, 6 mean-forcing balance.  The  */
+                           /* reference's uninitialised ufree is taken as 0.     */
   int32_t precision;       /* CF_PREC_FP64 or CF_PREC_FP32                       */
   int32_t device;          /* HIP device ordinal                                 */
   double alk;              /* SGS length of imodel 3; <= 0 selects dsl.  The    */

@@ -74,7 +74,8 @@ class Params:
     z0: float = 7.519e-4
     dt: float = 0.01
     imodel: int = 3           # 3 = Moeng84 TKE length, 4 = Smagorinsky (complen.f90:87-92)
-    isurface: int = 1         # 1 = Schumann wall model (lowerbc.f90:128-130)
+    isurface: int = 1         # wall model 1..6 (lowerbc.f90:126-161); 1 = Schumann
+    ufree: float = 0.0        # the uninitialised ufree of lowerbc.f90:38,63 (see wall_stress)
     alk: float | None = None  # SGS length scale (allocate.f90:61, never set; None -> dsl)
     ugal: float = 0.0         # storage module globals, zero in the reference run
     vgal: float = 0.0
@@ -318,28 +319,94 @@ def upper(st: State, pr: Params):
 
 
 # ----------------------------------------------------------------------------
-# Wall model (lowerbc.f90:25-165, upperbc.f90:210-337), isurface = 1
+# Wall models (lowerbc.f90:25-165 sufsim, upperbc.f90:33-158 sufsim_upper)
 # ----------------------------------------------------------------------------
-def wall_stress(pr: Params, ub1, vb1, upper_wall: bool, st: State):
-    """Schumann's wall stress on the 3/2 grid from the level-1 (or nnz) plane.
+BETAA = 0.35                 # parameters.f90:8
 
-    Returns (tauuw, tauvw, utau, abort_flag).  lowerbc.f90:57-67,83-91,128-130;
-    upperbc.f90:241-251,259-267,303-305.  The upper wall adds ugal to the plane
-    mean (upperbc.f90:241-242)."""
-    u1xy = float(np.sum(ub1)) * pr.xymb
-    v1xy = float(np.sum(vb1)) * pr.xymb
+
+def _fmax(a, b):
+    """Fortran MAX(a, b) as the reference's compiler evaluates it, a > b ? a : b
+    (so a NaN second argument wins)."""
+    return a if a > b else b
+
+
+def wall_stress(pr: Params, ub1, vb1, upper_wall: bool, st: State):
+    """Wall stress on the 3/2 grid from the level-1 (or nnz) plane.
+
+    Returns (tauuw, tauvw, utau, abort_flag).  Scalars: lowerbc.f90:56-91,
+    upperbc.f90:63-88 (the upper wall adds ugal to the plane mean).  The
+    fluctuation statistics lowerbc.f90:95-123 / upperbc.f90:93-121 and the six
+    models lowerbc.f90:126-161 / upperbc.f90:124-156.  Decisions where the
+    reference reads undefined values:
+      - ufree (uninitialised local, lowerbc.f90:38,63) is Params.ufree, 0 by
+        default, so windm = max(<wind>, 0.01).  A NaN ufree reproduces what
+        the compiled reference does (windm = 0.01);
+      - case 2 reads ufluc, vfluc, which the reference computes only for
+        isurface > 2; they are computed as defined (lowerbc.f90:97-98) for
+        every case;
+      - moengv2m is computed from moengu2, as the reference does
+        (lowerbc.f90:121, upperbc.f90:119).
+    Case 6 uses st.utau_s, st.dudt, st.dvdt of the lower wall's call."""
+    N = ub1.size
+    u1xy = float(np.sum(ub1)) / N
+    v1xy = float(np.sum(vb1)) / N
     if upper_wall:
         u1xy += pr.ugal
         v1xy += pr.vgal
+    wind = np.sqrt(ub1 ** 2 + vb1 ** 2)
+    windm = float(np.sum(wind)) / N
+    windm = _fmax(windm, pr.ufree)
+    windm = _fmax(windm, 0.01)
     vsfc = math.sqrt(u1xy * u1xy + v1xy * v1xy)
     vsfc = max(vsfc, 0.01)
     utau = vsfc * VK / pr.zody
+    tep = min(max(u1xy / vsfc, -1.0), 1.0)
+    thta = math.acos(tep)
     utau2 = utau * utau
-    if pr.isurface != 1:
-        raise NotImplementedError("oracle implements isurface=1 (Schumann) only")
-    sgn = 1.0 if upper_wall else -1.0
-    tuw = sgn * utau2 / vsfc * (ub1 + pr.ugal)
-    tvw = sgn * utau2 / vsfc * (vb1 + pr.vgal)
+    sgn_v = 1.0 if v1xy >= 0 else -1.0               # sign(1., v1xy)
+    if upper_wall:
+        auuwm = utau2 * math.cos(thta)
+        auvwm = utau2 * math.sin(thta) * sgn_v
+    else:
+        auuwm = -utau2 * math.cos(thta)
+        auvwm = -utau2 * math.sin(thta) * sgn_v
+    sfluc = wind - windm
+    ufluc = ub1 + pr.ugal - u1xy
+    vfluc = vb1 + pr.vgal - v1xy
+    ufluc2mroot = math.sqrt(float(np.sum(ufluc ** 2)) / N)
+    vfluc2mroot = math.sqrt(float(np.sum(vfluc ** 2)) / N)
+    suflucm = float(np.sum(ufluc * sfluc)) / N
+    svflucm = float(np.sum(vfluc * sfluc)) / N
+    moengu = sfluc * u1xy + windm * ufluc
+    moengv = sfluc * v1xy + windm * vfluc
+    moengu2mroot = math.sqrt(float(np.sum(moengu ** 2)) / N)
+    moengv2mroot = moengu2mroot                       # moengv2m = <moengu2>
+    sg = 1.0 if upper_wall else -1.0
+    c = pr.isurface
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if c == 1:                                    # Schumann
+            tuw = sg * utau2 / vsfc * (ub1 + pr.ugal)
+            tvw = sg * utau2 / vsfc * (vb1 + pr.vgal)
+        elif c == 2:                                  # Moeng
+            tuw = sg * utau2 / (windm * vsfc) * (windm * ufluc + wind * u1xy)
+            tvw = sg * utau2 / (windm * vsfc) * (windm * vfluc + wind * v1xy)
+        elif c == 3:                                  # Wei, Schumann fluctuation
+            tuw = auuwm - sg * BETAA * ufluc * utau2 / ufluc2mroot
+            tvw = auvwm - sg * BETAA * vfluc * utau2 / vfluc2mroot
+        elif c == 4:                                  # Moeng without linearisation
+            denom = math.sqrt((windm * u1xy + suflucm) ** 2 + (windm * v1xy + svflucm) ** 2)
+            tuw = sg * utau2 * (wind * (ub1 + pr.ugal)) / denom
+            tvw = sg * utau2 * (wind * (vb1 + pr.vgal)) / denom
+        elif c == 5:                                  # Wei, Moeng fluctuation
+            tuw = auuwm - sg * BETAA * utau2 * moengu / moengu2mroot
+            tvw = auvwm - sg * BETAA * utau2 * moengv / moengv2mroot
+        elif c == 6:                                  # mean-forcing balance + Schumann
+            a6u = -pr.zl / 2 * (FORCING - st.dudt)
+            a6v = -pr.zl / 2 * (-st.dvdt)
+            tuw = -sg * a6u + sg * utau * st.utau_s * ufluc / vsfc
+            tvw = -sg * a6v + sg * utau * st.utau_s * vfluc / vsfc
+        else:
+            raise NotImplementedError("isurface must be 1..6 (lowerbc.f90:126-161)")
     return tuw, tvw, utau, utau > 10.0
 
 

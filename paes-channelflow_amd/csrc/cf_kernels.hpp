@@ -59,8 +59,13 @@ struct WallScalars {
   double c_lo, c_hi;          // utau^2/vsfc of the lower and upper wall
   double utau_lo, utau_hi;
   double utau, utau_s, dudt, dvdt;
+  double mu[2], mv[2];        // plane means of u, v at level 1 / nnz (without ugal)
+  double vsfc[2];             // max(|<u> + ugal|, 0.01) per wall
+  // tau_uw, tau_vw at the wall = A + B q + (C + D q) wind with q = u or v
+  // at the wall level, wind = sqrt(u^2 + v^2): wc[wall][0: u, 1: v][A B C D]
+  double wc[2][2][4];
   int flags;                  // bit 0: utau > 10, bit 1: e clamp, bit 2: non-finite
-  int pad;
+  int wind_terms;             // C or D nonzero
 };
 
 enum : int { FLAG_UTAU = 1, FLAG_ECLAMP = 2, FLAG_NONFINITE = 4 };
@@ -73,6 +78,8 @@ struct Consts {
   double xymb, zody, vk, zl;
   double dsl;                 // filter width (dx dy dz)^(1/3), les.F90:118
   int nz, istage, mnout, zchunk;
+  int isurface;               // wall model 1..6 (lowerbc.f90:126-161)
+  int imodel;                 // 3: K from e; 4: K from the face shear (k_shear)
   int sgs_momentum;           // 0: interior SGS momentum stresses off (small-grid closure)
 };
 
@@ -125,6 +132,114 @@ struct RedSize {
 };
 
 // ---------------------------------------------------------------------------
+// Wall models (sufsim lowerbc.f90:25-165, sufsim_upper upperbc.f90:33-158)
+// as coefficients of tau = A + B q + (C + D q) wind.  Wall 0 is the lower
+// (level 1, signs -), wall 1 the upper (level nnz, signs +, ugal added to
+// the plane mean).  The fluctuation statistics of cases 2..5 come as shifted
+// plane sums st[8] over the N points of the 3/2 plane, with u' = u - <u>,
+// v' = v - <v>, s' = wind - s0, s0 = |(<u>, <v>)|:
+//   st = sum s', s'^2, u'^2, v'^2, s'u', s'v', u', v'.
+// Undefined values of the reference: ufree (lowerbc.f90:38,63) is taken as
+// 0; case 2's ufluc, vfluc (set only for isurface > 2) as defined
+// (lowerbc.f90:97-98); moengv2m = <moengu^2> as the reference computes it
+// (lowerbc.f90:121).
+// ---------------------------------------------------------------------------
+constexpr double BETAA = 0.35;                                 // parameters.f90:8
+
+__device__ inline void wall_coeffs(WallScalars* ws, const Consts& c, int wall, const double* st,
+                                   double N) {
+  const double sg = wall ? 1.0 : -1.0;
+  const double ugal = c.ugal, vgal = c.vgal;
+  const double u1xy = ws->mu[wall] + (wall ? ugal : 0.0);
+  const double v1xy = ws->mv[wall] + (wall ? vgal : 0.0);
+  const double vsfc = ws->vsfc[wall];
+  const double utau = wall ? ws->utau_hi : ws->utau_lo;
+  const double utau2 = utau * utau;
+  const double dU = ugal - u1xy, dV = vgal - v1xy;             // ufluc = u + dU
+  double* A = ws->wc[wall][0];
+  double* B = ws->wc[wall][1];
+  for (int i = 0; i < 4; ++i) A[i] = B[i] = 0.0;
+  switch (c.isurface) {
+    case 1: {                                                   // Schumann
+      const double k = sg * utau2 / vsfc;
+      A[0] = k * ugal; A[1] = k;
+      B[0] = k * vgal; B[1] = k;
+      return;
+    }
+    case 6: {                                                   // mean-forcing balance
+      const double a6u = -c.zl / 2.0 * (0.016 - ws->dudt), a6v = -c.zl / 2.0 * (-ws->dvdt);
+      const double r = sg * utau * ws->utau_s / vsfc;
+      A[0] = -sg * a6u + r * dU; A[1] = r;
+      B[0] = -sg * a6v + r * dV; B[1] = r;
+      return;
+    }
+    default: break;
+  }
+  const double mu = ws->mu[wall], mv = ws->mv[wall];
+  const double s0 = sqrt(mu * mu + mv * mv);
+  const double ms = st[0] / N, mss = st[1] / N, muu = st[2] / N, mvv = st[3] / N;
+  const double msu = st[4] / N, msv = st[5] / N, mu_ = st[6] / N, mv_ = st[7] / N;
+  double windm = s0 + ms;
+  windm = windm > 0.0 ? windm : 0.0;                           // ufree
+  windm = windm > 0.01 ? windm : 0.01;
+  const double eu = mu + dU, ev = mv + dV;                     // ufluc = u' + eu
+  const double ds = windm - s0;                                // sfluc = s' - ds
+  const double ufl2 = muu + 2.0 * eu * mu_ + eu * eu;
+  const double vfl2 = mvv + 2.0 * ev * mv_ + ev * ev;
+  const double sufl = msu - ds * mu_ + eu * ms - eu * ds;
+  const double svfl = msv - ds * mv_ + ev * ms - ev * ds;
+  const double moeng2 = u1xy * u1xy * (mss - 2.0 * ds * ms + ds * ds) + 2.0 * u1xy * windm * sufl +
+                        windm * windm * ufl2;
+  double tep = u1xy / vsfc;
+  tep = tep < 1.0 ? tep : 1.0;
+  tep = tep > -1.0 ? tep : -1.0;
+  const double thta = acos(tep);
+  const double auuwm = sg * utau2 * cos(thta);
+  const double auvwm = sg * utau2 * sin(thta) * (v1xy >= 0.0 ? 1.0 : -1.0);
+  switch (c.isurface) {
+    case 2: {                                                   // Moeng
+      const double k = sg * utau2 / (windm * vsfc);
+      A[0] = k * windm * dU; A[1] = k * windm; A[2] = k * u1xy;
+      B[0] = k * windm * dV; B[1] = k * windm; B[2] = k * v1xy;
+      break;
+    }
+    case 3: {                                                   // Wei, Schumann fluctuation
+      const double qu = -sg * BETAA * utau2 / sqrt(ufl2), qv = -sg * BETAA * utau2 / sqrt(vfl2);
+      A[0] = auuwm + qu * dU; A[1] = qu;
+      B[0] = auvwm + qv * dV; B[1] = qv;
+      break;
+    }
+    case 4: {                                                   // Moeng, not linearised
+      const double du = windm * u1xy + sufl, dv = windm * v1xy + svfl;
+      const double k = sg * utau2 / sqrt(du * du + dv * dv);
+      A[2] = k * ugal; A[3] = k;
+      B[2] = k * vgal; B[3] = k;
+      break;
+    }
+    case 5: {                                                   // Wei, Moeng fluctuation
+      const double q = -sg * BETAA * utau2 / sqrt(moeng2);
+      A[0] = auuwm + q * windm * (dU - u1xy); A[1] = q * windm; A[2] = q * u1xy;
+      B[0] = auvwm + q * windm * (dV - v1xy); B[1] = q * windm; B[2] = q * v1xy;
+      break;
+    }
+    default: break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void wall_tau(const WallScalars* ws, int wall, T u, T v, T& tu, T& tv) {
+  const double* A = ws->wc[wall][0];
+  const double* B = ws->wc[wall][1];
+  tu = (T)A[0] + (T)A[1] * u;
+  tv = (T)B[0] + (T)B[1] * v;
+  if (ws->wind_terms) {
+    const T s = sqrt(u * u + v * v);
+    tu += ((T)A[2] + (T)A[3] * u) * s;
+    tv += ((T)B[2] + (T)B[3] * v) * s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_wall: wall-model scalars.  One workgroup of 64 threads.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -166,6 +281,48 @@ __global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const 
   ws->dvdt = sv / nz;
   ws->utau_s = sqrt(c.zl / 2.0 * sqrt(ws->dvdt * ws->dvdt + (0.016 - ws->dudt) * (0.016 - ws->dudt)));
   if (!(ut <= 10.0) || !(ut2 <= 10.0)) ws->flags |= FLAG_UTAU;  // lowerbc.f90:77-80
+  ws->mu[0] = u1; ws->mv[0] = v1;
+  ws->mu[1] = (double)U[(size_t)nz * plane].x;
+  ws->mv[1] = (double)V[(size_t)nz * plane].x;
+  ws->vsfc[0] = vs; ws->vsfc[1] = vs2;
+  ws->wind_terms = (c.isurface == 2 || c.isurface == 4 || c.isurface == 5) ? 1 : 0;
+  if (c.isurface == 1 || c.isurface == 6) {                   // no plane statistics needed
+    wall_coeffs(ws, c, 0, nullptr, 1.0);
+    wall_coeffs(ws, c, 1, nullptr, 1.0);
+  }
+}
+
+// Plane statistics of the wall-level 3/2-grid velocity for isurface 2..5:
+// per row y of level 1 (wall 0) or nnz (wall 1), the sums of wall_coeffs'
+// st[8] into part[(wall*NYB + y)*8 + q].  grid (NYB, 2), one thread per point.
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_wall_plane(const cx<T>* __restrict__ Bs, size_t bstride,
+                                                           const T* __restrict__ kxp,
+                                                           const cx<T>* __restrict__ twNXB, int nz,
+                                                           const WallScalars* __restrict__ ws,
+                                                           double* part);
+
+// Sums of k_wall_plane's rows, then wall_coeffs for both walls.  One
+// workgroup of 256 threads.
+__global__ void __launch_bounds__(256) k_wall_final(const double* __restrict__ part, int nyb, double npts,
+                                                    Consts c, WallScalars* ws) {
+  __shared__ double red[16 * 4];
+  const int tid = threadIdx.x;
+  double s[16];
+  for (int q = 0; q < 16; ++q) s[q] = 0.0;
+  for (int y = tid; y < nyb; y += 256)
+    for (int w = 0; w < 2; ++w)
+      for (int q = 0; q < 8; ++q) s[w * 8 + q] += part[((size_t)w * nyb + y) * 8 + q];
+  for (int q = 0; q < 16; ++q)
+    for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off, 64);
+  if ((tid & 63) == 0)
+    for (int q = 0; q < 16; ++q) red[q * 4 + (tid >> 6)] = s[q];
+  __syncthreads();
+  if (tid != 0) return;
+  double st[16];
+  for (int q = 0; q < 16; ++q) st[q] = red[q * 4] + red[q * 4 + 1] + red[q * 4 + 2] + red[q * 4 + 3];
+  wall_coeffs(ws, c, 0, st, npts);
+  wall_coeffs(ws, c, 1, st + 8, npts);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,6 +433,7 @@ struct RowArgs {
   double* part;            // divz and SGS partial sums [7][slots][NYB]
   size_t pstride;          // slots*NYB
   int* flags;
+  const T* X;              // imodel 4: K at faces 0..nnz [face][NYB][NXB] (k_shear)
   Consts c;
 };
 
@@ -387,6 +545,106 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
 }
 
 template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_wall_plane(const cx<T>* __restrict__ Bs, size_t bstride,
+                                                           const T* __restrict__ kxp,
+                                                           const cx<T>* __restrict__ twNXB, int nz,
+                                                           const WallScalars* __restrict__ ws,
+                                                           double* part) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT, NYB = NXB;
+  __shared__ cx<T> buf[6 * padlen(NXB)];
+  __shared__ cx<T> tw[NXB];
+  __shared__ double red[RedSize<NT, 8>::N];
+  const int tid = threadIdx.x, y = blockIdx.x, wall = blockIdx.y;
+  const int L = wall ? nz : 1;
+  load_twiddles(tw, twNXB, NXB, tid, NT);
+  __syncthreads();
+  const size_t row = (size_t)L * NYB * MX + (size_t)y * MX;
+  const cx<T>* sa[1] = {Bs + row};                      // u (field 0)
+  const cx<T>* sb[1] = {Bs + 2 * bstride + row};        // v (field 2)
+  const bool da[1] = {false}, db[1] = {false};
+  T ra[1], rb[1];
+  synth_pairs<T, NX, 1>(buf, sa, da, sb, db, ra, rb, kxp, tw, tid);
+  const double mu = ws->mu[wall], mv = ws->mv[wall];
+  const double s0 = sqrt(mu * mu + mv * mv);
+  const T uT = ra[0], vT = rb[0];
+  const double sp = (double)sqrt(uT * uT + vT * vT) - s0;   // lowerbc.f90:60 in field precision
+  const double up = (double)uT - mu, vp = (double)vT - mv;
+  double q[8] = {sp, sp * sp, up * up, vp * vp, sp * up, sp * vp, up, vp};
+  block_sum<NT, 8>(q, red, tid);
+  if (tid < 8) part[((size_t)wall * NYB + y) * 8 + tid] = q[0];
+}
+
+// imodel 4 (complen.f90:13-17,22-23): the eddy diffusivity at faces 0..nnz
+// from the face shear
+//   S(f) = G(f) + G(f+1) + (uz(f) + wx(f))^2 + (vz(f) + wy(f))^2,
+//   G(L) = ux^2 + vy^2 + wz^2 + 0.5 (uy + vx)^2 at level L,
+// with the ghost values of derivsbot/derivs (derivs1.f90:29-38,
+// derivs2.f90:22-32): G(0) = G(nnz+1) = 0, uz(0) = uz(1),
+// uz(nnz) = uz(nnz-1), wx = wy = 0 at faces 0 and nnz; wz(L) =
+// (w(L) - w(L-1))/dz with w = 0 at faces 0 and nnz.
+// K = min((0.1 dsl)^2 sqrt(S), xkmax) into X[f][y][x].
+// grid (NYB, chunks of c.zchunk faces), one thread per point.  A chunk
+// starts one level below its first face, so every face it writes sees the
+// levels on both sides.
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_shear(const cx<T>* __restrict__ Bs, size_t bstride,
+                                                      const T* __restrict__ kxp,
+                                                      const cx<T>* __restrict__ twNXB, Consts c, T* X) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT, NYB = NXB;
+  __shared__ cx<T> buf[6 * padlen(NXB)];
+  __shared__ cx<T> tw[NXB];
+  const int tid = threadIdx.x, y = blockIdx.x;
+  const int nz = c.nz;
+  const int f0 = blockIdx.y * c.zchunk;
+  const int f1 = (f0 + c.zchunk < nz + 1) ? f0 + c.zchunk : nz + 1;
+  load_twiddles(tw, twNXB, NXB, tid, NT);
+  __syncthreads();
+  const size_t plane = (size_t)NYB * MX;
+  auto row = [&](int fld, int L) -> const cx<T>* {
+    return Bs + (size_t)fld * bstride + (size_t)L * plane + (size_t)y * MX;
+  };
+  auto face = [&](int fld, int L) -> const cx<T>* { return (L >= 1 && L <= nz - 1) ? row(fld, L) : nullptr; };
+  const T dzm = (T)c.dzm;
+  const T kc = (T)((0.1 * c.dsl) * (0.1 * c.dsl)), xkmax = (T)c.xkmax;
+  T* Xo = X + (size_t)y * NXB + tid;
+  const size_t pp = (size_t)NYB * NXB;
+  auto emit = [&](int f, T S) {
+    if (f >= f0 && f < f1) {
+      const T k = kc * sqrt(S);
+      Xo[(size_t)f * pp] = k < xkmax ? k : xkmax;
+    }
+  };
+  enum { FU = 0, FUY, FV, FVY, FW, FWY };
+  const int Ls = f0 - 1 > 1 ? f0 - 1 : 1;
+  const int Le = f1 < nz ? f1 : nz;
+  T Gp = 0, up = 0, vp = 0, wxp = 0, wyp = 0, wprev = 0;   // previous level
+  for (int L = Ls; L <= Le; ++L) {
+    T ra[5], rb[5];
+    const cx<T>* sa[5] = {row(FU, L), row(FUY, L), row(FU, L), face(FW, L), face(FWY, L)};
+    const cx<T>* sb[5] = {row(FV, L), row(FVY, L), row(FV, L), face(FW, L), nullptr};
+    const bool da[5] = {true, false, false, false, false};
+    const bool db[5] = {true, false, false, true, false};
+    synth_pairs<T, NX, 5>(buf, sa, da, sb, db, ra, rb, kxp, tw, tid);
+    const T ux = ra[0], vx = rb[0], uy = ra[1], vy = rb[1], u = ra[2], v = rb[2];
+    const T w = ra[3], wx = rb[3], wy = ra[4];
+    const T wz = (w - wprev) * dzm;
+    const T sxy = uy + vx;
+    const T G = ux * ux + vy * vy + wz * wz + T(0.5) * sxy * sxy;
+    if (L >= 2) {
+      const T uz = (u - up) * dzm, vz = (v - vp) * dzm;         // face L-1
+      const T a1 = uz + wxp, a2 = vz + wyp;
+      emit(L - 1, Gp + G + a1 * a1 + a2 * a2);
+      if (L == 2) emit(0, Gp + uz * uz + vz * vz);
+      if (L == nz) emit(nz, G + uz * uz + vz * vz);
+    }
+    Gp = G; up = u; vp = v; wxp = wx; wyp = wy; wprev = w;
+    __syncthreads();                                            // buf is rewritten next level
+  }
+}
+
+template <typename T, int NX>
 __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   using F = RowFFT<T, NX>;
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
@@ -414,7 +672,6 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   const T alk = (T)c.alk, xkmax = (T)c.xkmax;
   const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
-  const T c_lo = (T)a.ws->c_lo, c_hi = (T)a.ws->c_hi;
   const bool mn = c.mnout != 0;
   const bool sgs_mom = c.sgs_momentum != 0;
   int clampflag = 0;
@@ -502,7 +759,16 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     double sums[7];
     const bool top = (iz == nz);
     {
-      const T xm = xkk_of(em), x0 = xkk_of(e0), xp = xkk_of(ep);
+      T xm, x0, xp;
+      if (c.imodel == 4) {                                        // K from k_shear
+        const T* Xr = a.X + (size_t)y * NXB + tid;
+        const size_t pp = (size_t)NYB * NXB;
+        xm = Xr[(size_t)(iz - 1) * pp];
+        x0 = Xr[(size_t)iz * pp];
+        xp = top ? x0 : Xr[(size_t)(iz + 1) * pp];             // unused at the top
+      } else {
+        xm = xkk_of(em); x0 = xkk_of(e0); xp = xkk_of(ep);
+      }
       const T xs = xm + x0;
       T tuu = -xs * ux;                                           // compsgs1.f90:144-146
       T tvv = -xs * vy;
@@ -512,16 +778,14 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
       T tww0 = -xs * wz0;                                         // compsgs1.f90:129,151
       T twwp = -(x0 + xp) * wzp;
       T tuw0, tvw0, tuwm, tvwm;
-      if (top) {                                                  // upperbc.f90:304-305
-        tuw0 = c_hi * (u0 + ugal);
-        tvw0 = c_hi * (v0 + vgal);
+      if (top) {                                                  // sufsim_upper
+        wall_tau(a.ws, 1, u0, v0, tuw0, tvw0);
       } else {                                                    // compsgs1.f90:152-153
         tuw0 = -x0 * ((up - u0) * dzm + wx0);
         tvw0 = -x0 * ((vp - v0) * dzm + wy0);
       }
-      if (iz == 1) {                                              // lowerbc.f90:129-130
-        tuwm = -c_lo * (u0 + ugal);
-        tvwm = -c_lo * (v0 + vgal);
+      if (iz == 1) {                                              // sufsim
+        wall_tau(a.ws, 0, u0, v0, tuwm, tvwm);
       } else {
         tuwm = -xm * ((u0 - um) * dzm + wxm);
         tvwm = -xm * ((v0 - vm) * dzm + wym);
@@ -672,7 +936,6 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
   const T alk = (T)c.alk, xkmax = (T)c.xkmax;
   const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
-  const T c_lo = (T)a.ws->c_lo, c_hi = (T)a.ws->c_hi;
   const float dslf = (float)c.dsl, dzf = (float)c.dz, zlf = (float)c.zl;
   const bool mn = c.mnout != 0;
   int clampflag = 0;
@@ -777,9 +1040,8 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
     const T xm = xkk_of(em), x0 = xkk_of(e0), xp = xkk_of(ep);
     // final face-iz stresses (carried to the next level as face iz-1)
     T tuw0, tvw0;
-    if (top) {                                                    // upperbc.f90:304-305
-      tuw0 = c_hi * (u0 + ugal);
-      tvw0 = c_hi * (v0 + vgal);
+    if (top) {                                                    // sufsim_upper
+      wall_tau(a.ws, 1, u0, v0, tuw0, tvw0);
     } else if (ok0 && ok1) {
       tuw0 = (T)(0.5f * (t0[4] + t1[4]));
       tvw0 = (T)(0.5f * (t0[5] + t1[5]));
@@ -808,9 +1070,8 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
         }
         if (ok1) twwp = (T)(t1[2] - (t1[0] + t1[1] + t1[2]) * (1.0f / 3.0f));
         T tuwm, tvwm;
-        if (iz == 1) {                                            // lowerbc.f90:129-130
-          tuwm = -c_lo * (u0 + ugal);
-          tvwm = -c_lo * (v0 + vgal);
+        if (iz == 1) {                                            // sufsim
+          wall_tau(a.ws, 0, u0, v0, tuwm, tvwm);
         } else {
           tuwm = fuw;
           tvwm = fvw;

@@ -88,6 +88,8 @@ struct cf_ctx {
   void* sgsS = nullptr;         // small-grid closure: 6 x-spectral input rows (band-sized each)
   void* sgsO = nullptr;         // and its 6 output rows
   double* sred = nullptr;       // its SGS means [6][slots]
+  double* wpart = nullptr;      // isurface 2..5: wall-plane sums [2][NYB][8]
+  void* X = nullptr;            // imodel 4: K at faces 0..nnz [nz+1][NYB][NXB]
   bool have_sred = false;       // sred holds the last mnout stage's means
   // box-filter work buffers (allocated on first use)
   double* filt_in = nullptr;    // u, v, w, e physical, 4 * nz*ny*nx
@@ -190,6 +192,10 @@ struct EngineT : Engine {
     if ((rc = dalloc(ctx, &ctx->kyd, (size_t)NY)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->staging, (size_t)ctx->nz * NY * NX)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->ws, (size_t)1)) != CF_OK) return rc;
+    if (ctx->p.isurface >= 2 && ctx->p.isurface <= 5)      // wall-plane statistics
+      if ((rc = dalloc(ctx, &ctx->wpart, (size_t)2 * NYB * 8)) != CF_OK) return rc;
+    if (ctx->p.imodel == 4)                                 // K at faces 0..nnz on the 3/2 grid
+      if ((rc = dalloc(ctx, (T**)&ctx->X, (size_t)(ctx->nz + 1) * NYB * NXB)) != CF_OK) return rc;
     // tables
     auto t1 = twiddles<T>(NX), t2 = twiddles<T>(NXB);
     auto t3 = twiddles<double>(NX);
@@ -224,6 +230,8 @@ struct EngineT : Engine {
     c.alk = p.alk > 0 ? p.alk : dsl;
     c.dsl = dsl;
     c.sgs_momentum = (ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL) ? 0 : 1;
+    c.isurface = ctx->p.isurface;
+    c.imodel = ctx->p.imodel;
     c.xkmax = dz * dz / ctx->dt / 5.0;                                  // les.F90:216
     c.ceps = 0.93 / dsl;                                                // les.F90:120
     c.emin = 1.0e-6;                                                    // les.F90:102
@@ -273,6 +281,19 @@ struct EngineT : Engine {
       hipLaunchKernelGGL((cf::k_col_synth<T, NX, NY>), dim3(MX / CS::TK, nz, 4), dim3(CS::NT), 0,
                          s, a);
     }
+    if (ctx->p.isurface >= 2 && ctx->p.isurface <= 5) {    // wall-plane statistics, then the wall coefficients
+      hipLaunchKernelGGL((cf::k_wall_plane<T, NX>), dim3(NYB, 2), dim3(RS::NT), 0, s, cptr(ctx->Bs), bstride,
+                         rptr(ctx->kxp), cptr(ctx->twNXB), nz, ctx->ws, ctx->wpart);
+      hipLaunchKernelGGL(cf::k_wall_final, dim3(1), dim3(256), 0, s, ctx->wpart, NYB, (double)NYB * NXB, c,
+                         ctx->ws);
+    }
+    if (ctx->p.imodel == 4) {                               // K from the face shear
+      cf::Consts cs = c;
+      cs.zchunk = ctx->zchunk;
+      const int nchs = (nz + 1 + ctx->zchunk - 1) / ctx->zchunk;
+      hipLaunchKernelGGL((cf::k_shear<T, NX>), dim3(NYB, nchs), dim3(RS::NT), 0, s, cptr(ctx->Bs), bstride,
+                         rptr(ctx->kxp), cptr(ctx->twNXB), cs, rptr(ctx->X));
+    }
     mark();
     {
       cf::RowArgs<T> a{};
@@ -286,6 +307,7 @@ struct EngineT : Engine {
       a.part = ctx->part;
       a.pstride = (size_t)ctx->slots * NYB;
       a.flags = &ctx->ws->flags;
+      a.X = rptr(ctx->X);
       a.c = c;
       bool learned = false;
       if constexpr ((3 * NX / 2) % 64 == 0) {
@@ -620,8 +642,8 @@ int cf_create(const cf_params* p, cf_ctx** out) {
     g_create_error = "nnz >= 4 required";
     return CF_E_ARG;
   }
-  if (p->imodel != 3 || p->isurface != 1) {
-    g_create_error = "device path implements imodel=3 and isurface=1";
+  if ((p->imodel != 3 && p->imodel != 4) || p->isurface < 1 || p->isurface > 6) {
+    g_create_error = "imodel must be 3 or 4 and isurface 1..6";
     return CF_E_ARG;
   }
   if (!(p->dt > 0) || !(p->xl > 0) || !(p->yl > 0) || !(p->zl > 0) || !(p->z0 > 0)) {
@@ -1357,6 +1379,10 @@ int cf_set_closure(cf_ctx* ctx, const cf_mlp* m) {
   }
   if ((ctx->nxb % 64) != 0) {
     ctx->err = "the learned closure needs nnx in {128, 256, 512} (3/2-grid rows of whole wavefronts)";
+    return CF_E_ARG;
+  }
+  if (ctx->p.imodel != 3) {
+    ctx->err = "the learned closure falls back to the TKE closure: imodel 3 required";
     return CF_E_ARG;
   }
   if (m->device != ctx->p.device) {

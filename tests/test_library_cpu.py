@@ -33,7 +33,8 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("kw", [dict(nnx=48, nny=48), dict(nnx=64, nny=32), dict(nnz=2),
-                                dict(imodel=4), dict(isurface=6), dict(dt=0.0), dict(precision=7)])
+                                dict(imodel=5), dict(isurface=7), dict(isurface=0), dict(dt=0.0),
+                                dict(precision=7)])
 def test_create_rejects_unsupported_parameters(kw):
     import channelflow as C
     args = dict(nnx=32, nny=32, nnz=16)

@@ -38,9 +38,17 @@ def run(harness, payload, tmp_path):
     return np.fromfile(fout, dtype=np.float64)
 
 
-@pytest.mark.parametrize("imodel", [3, 4])
-def test_sgs_stresses_match_reference(harness, tmp_path, imodel):
-    pr = O.Params(16, 16, 8, dt=0.01, imodel=imodel)
+@pytest.mark.parametrize("imodel,isurface", [(3, 1), (4, 1), (3, 3), (3, 6), (4, 6), (3, 4), (3, 5)])
+def test_sgs_stresses_match_reference(harness, tmp_path, imodel, isurface):
+    """Cases 4 and 5 go through windm = max(max(<wind>, ufree), 0.01) with
+    ufree an uninitialised local (lowerbc.f90:38,63,64).  The compiled
+    reference gives windm = 0.01 there, as a NaN ufree does under a > b ? a : b,
+    so the oracle runs with ufree = NaN for this comparison; the product
+    takes ufree = 0.  Case 2 is not compared: it also reads ufluc/vfluc, which
+    the reference computes only for isurface > 2 (lowerbc.f90:95-99,131-132);
+    test_oracle_wall.py pins it by invariants."""
+    pr = O.Params(16, 16, 8, dt=0.01, imodel=imodel, isurface=isurface,
+                  ufree=float("nan") if isurface in (2, 4, 5) else 0.0)
     rng = np.random.default_rng(11)
     u, v, w, e = O.initial_fields(pr, seed=5)
     e = e * (1 + 0.5 * rng.uniform(-1, 1, e.shape))        # make xkk vary in x, y, z
