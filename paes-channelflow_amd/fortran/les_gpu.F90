@@ -13,7 +13,8 @@
 ! One optional namelist group is added:
 !   &GPU precision=0|1 (0: FP64, 1: FP32 fields with FP64 pressure),
 !        device, alk (<= 0: dsl), seed, amp,
-!        closure=0|1 (1: learned SGS closure with TKE fallback), weights='file' /
+!        closure=0|1 (1: learned SGS closure with TKE fallback), weights='file',
+!        closure_grid=0|1 (0: network on the 3/2 grid, 1: on the nnx x nny grid) /
 ! seed/amp select the seeded initial condition used when restart=.false.
 ! (the reference's initvel draws from the compiler's random_number).
 ! Restart files may hold REAL(4) records (the shipped reference) or REAL(8)
@@ -99,6 +100,11 @@ module channelflow_c
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx, m
      end function cf_set_closure
+     integer(c_int) function cf_set_closure_grid(ctx, grid) bind(C, name="cf_set_closure_grid")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: grid
+     end function cf_set_closure_grid
      integer(c_int) function cf_get_closure_counts(ctx, counts) bind(C, name="cf_get_closure_counts")
        import :: c_int, c_ptr, c_int64_t
        type(c_ptr), value :: ctx
@@ -145,7 +151,7 @@ program les_gpu
   character(100) :: restart_file, case
   integer :: timax, savfrq, imodel, isurface, nnx, nny, nnz
   real(8) :: meanfrq, cfl, dt, cb, skc, xl, yl, zl, z0
-  integer :: precision, device, closure
+  integer :: precision, device, closure, closure_grid
   integer(8) :: seed
   real(8) :: alk, amp
   character(256) :: weights
@@ -154,7 +160,7 @@ program les_gpu
        cb, skc, spectout, spectav, imodel, isurface
   namelist /GRIDSIZE/ nnx, nny, nnz
   namelist /PARAMS/ xl, yl, zl, z0
-  namelist /GPU/ precision, device, alk, seed, amp, closure, weights
+  namelist /GPU/ precision, device, alk, seed, amp, closure, weights, closure_grid
 
   type(cf_params) :: p
   type(c_ptr) :: ctx, net
@@ -173,7 +179,7 @@ program les_gpu
   nnx = 32; nny = 32; nnz = 32; xl = 5000.0d0; yl = 5000.0d0; zl = 2000.0d0; z0 = 0.16d0
   cfl = 0.3d0; dt = 0.01d0; cb = 0; skc = 0; imodel = 3; isurface = 1
   precision = CF_PREC_FP64; device = 0; alk = 0.0d0; seed = 10972_8; amp = 0.1d0
-  closure = 0; weights = "sgs_mlp.cfmlp"; net = c_null_ptr
+  closure = 0; weights = "sgs_mlp.cfmlp"; net = c_null_ptr; closure_grid = 0
   divgls = 0.0d0; fcor = 0.0001d0; ugtop = 15.0d0; ugbot = 15.0d0; vgtop = 0.0d0; vgbot = 0.0d0
 
   write(*,*)
@@ -227,6 +233,7 @@ program les_gpu
         stop 1
      end if
      call check(cf_set_closure(ctx, net), ctx, "cf_set_closure")
+     call check(cf_set_closure_grid(ctx, int(closure_grid, c_int32_t)), ctx, "cf_set_closure_grid")
      write(*,*) "Learned SGS closure from ", trim(weights), " (TKE fallback)"
   end if
 
