@@ -504,6 +504,16 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   }
 }
 
+// Element q (wave-uniform, not a constant) of a small local array, by
+// compare-and-select: indexing it directly would place the array in scratch.
+template <int NP, typename V>
+__device__ __forceinline__ V pick(const V* arr, int q, V dflt) {
+  V r = dflt;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) r = (j == q) ? arr[j] : r;
+  return r;
+}
+
 // Forward-transform NP real row pairs (this thread holds point tid of each)
 // and hand coefficient k < MX of both members, scaled by 1/NXB, to
 // emit(q, k, A, B).
@@ -1270,8 +1280,8 @@ __device__ __forceinline__ void small_prefetch(SmallPre<T, NX>& P, const cx<T>* 
 #pragma unroll
   for (int qq = 0; qq < F::TPW; ++qq) {
     const int q = w * F::TPW + qq;
-    const cx<T>* pa = q < NP ? sa[q] : nullptr;
-    const cx<T>* pb = q < NP ? sb[q] : nullptr;
+    const cx<T>* pa = pick<NP>(sa, q, (const cx<T>*)nullptr);
+    const cx<T>* pb = pick<NP>(sb, q, (const cx<T>*)nullptr);
 #pragma unroll
     for (int kk = 0; kk < SmallPre<T, NX>::KPL; ++kk) {
       const int k = lane + 64 * kk;
@@ -1294,7 +1304,7 @@ __device__ __forceinline__ void synth_small_pre(cx<T>* buf, const SmallPre<T, NX
     const int q = w * F::TPW + qq;
     if (q < NP) {
       cx<T>* z = buf + q * padlen(NX);
-      const bool dqa = da[q], dqb = db[q];
+      const bool dqa = pick<NP>(da, q, false), dqb = pick<NP>(db, q, false);
 #pragma unroll
       for (int kk = 0; kk < SmallPre<T, NX>::KPL; ++kk) {
         const int k = lane + 64 * kk;
