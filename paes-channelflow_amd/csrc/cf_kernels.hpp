@@ -1695,37 +1695,15 @@ struct AnalArgs {
   int nz;
 };
 
-// Load the flux columns of tile t (see k_col_anal) into this thread's registers.
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void anal_load_tile(const AnalArgs<T>& a, int t,
-                                               cx<T> (&pre)[2 * ColShape<T, NX, NY>::TK * (3 * NY / 2) / 256],
-                                               int tid) {
-  using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
-  constexpr int NKT = MX / TK;
-  constexpr int LPT = 2 * TK * NYB / NT;
-  const int kt = t % NKT, rest = t / NKT;
-  const int z = rest % a.nz + 1, cmp = rest / a.nz;
-  if (cmp >= 2 && z == a.nz) return;
-  const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + kt * TK;
-#pragma unroll
-  for (int i = 0; i < LPT; ++i) {
-    const int idx = tid + i * NT;
-    const int c = idx % TK, r = idx / TK;
-    const int n = r % NYB, d = r / NYB;
-    pre[i] = g[(size_t)d * a.astride + (size_t)n * MX + c];
-  }
-}
-
-// Persistent: each workgroup walks tiles (kx tile fastest, then level, then
-// component), holding the next tile's columns in registers while the current
-// tile's transforms run.
+// One tile per workgroup: TK kx columns of one level and component.  (A
+// persistent loop over tiles with the next tile's columns held in registers
+// took 230-400 VGPRs, one wave per SIMD, and measured 37 % slower.)
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
   constexpr int NKT = MX / TK;
-  constexpr int LPT = 2 * TK * NYB / NT;       // prefetched elements per thread
+  constexpr int LPT = 2 * TK * NYB / NT;       // tile elements per thread
   static_assert((2 * TK * NYB) % NT == 0, "tile must divide over the workgroup");
   constexpr int PLB = padlen(NYB), PLS = padlen(NY);
   const int tid = threadIdx.x;
@@ -1738,29 +1716,26 @@ __global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
   const int ntiles = NKT * nz * 4;
   load_twiddles(twb, a.twNYB, NYB, tid, NT);
   load_twiddles(tws, a.twNY, NY, tid, NT);
-  cx<T> pre[LPT];
-  int t = blockIdx.x;
-  if (t < ntiles) anal_load_tile<T, NX, NY>(a, t, pre, tid);
-  for (; t < ntiles; t += gridDim.x) {
+  const int t = blockIdx.x;
+  if (t < ntiles) {
     const int kt = t % NKT, rest = t / NKT;
     const int z = rest % nz + 1, cmp = rest / nz;
     const int k0 = kt * TK;
-    const int tn = t + gridDim.x;
     if (cmp >= 2 && z == nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
       cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
       for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
-      if (tn < ntiles) anal_load_tile<T, NX, NY>(a, tn, pre, tid);
-      continue;
+      return;
     }
-    __syncthreads();                           // the previous tile is out of LDS
+    {
+      const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + k0;
 #pragma unroll
-    for (int i = 0; i < LPT; ++i) {
-      const int idx = tid + i * NT;
-      const int c = idx % TK, r = idx / TK;
-      const int n = r % NYB, d = r / NYB;
-      buf[(d * TK + c) * PLB + pidx(n)] = pre[i];
+      for (int i = 0; i < LPT; ++i) {
+        const int idx = tid + i * NT;
+        const int c = idx % TK, r = idx / TK;
+        const int n = r % NYB, d = r / NYB;
+        buf[(d * TK + c) * PLB + pidx(n)] = g[(size_t)d * a.astride + (size_t)n * MX + c];
+      }
     }
-    if (tn < ntiles) anal_load_tile<T, NX, NY>(a, tn, pre, tid);   // in flight during the transforms
     __syncthreads();
     fft_waves<T, NYB, false, 2 * TK, NT / 64>(buf, twb, tid);
     __syncthreads();

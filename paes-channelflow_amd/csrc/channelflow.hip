@@ -336,9 +336,8 @@ struct EngineT : Engine {
       a.twNY = cptr(ctx->twNX);
       a.twNYB = cptr(ctx->twNXB);
       a.nz = nz;
-      const int ntiles = (MX / CS::TK) * nz * 4;
-      const int grid = std::min(ntiles, 2 * ctx->ncu);
-      hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(grid), dim3(CS::NT), 0, s, a);
+      const int ntiles = (MX / CS::TK) * nz * 4;      // tiles: kx fastest, then level, then component
+      hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(ntiles), dim3(CS::NT), 0, s, a);
     }
     mark();
     bool small = false;
