@@ -83,7 +83,30 @@ MEASURED_TRAFFIC = {
     ("k_rows", 256, 128, "fp32"): round((2 * 198014.8 + 393262.8) * 1024),
     # profiles/k_rows_mlp_counters_256x256x128.md
     ("k_rows_mlp", 256, 128, "fp32"): round((2 * 202209 + 393792) * 1024),
+    # profiles/learned_small_grid_counters_256x256x128.md
+    ("k_sgs_rows", 256, 128, "fp32"): round((2 * 122637.4 + 196864.0) * 1024),
 }
+
+# SQ_INSTS_VALU per dispatch (wavefront instructions) at 256x256x128 FP32, from
+# the same counter profiles.  The VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz,
+# one wave64 VALU instruction per 4 cycles per SIMD (16 lanes; no packed FP32).
+MEASURED_VALU = {
+    ("k_rows", 256, 128, "fp32"): 241.24e6,
+    ("k_rows_mlp", 256, 128, "fp32"): 751.0e6,
+    ("k_sgs_rows", 256, 128, "fp32"): 303.04e6,
+}
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def valu_roofline(kernel, args, ms):
+    """The dominant kernel against the VALU issue peak: its SQ_INSTS_VALU per
+    launch (counter profile) / its launch time.  None where not profiled."""
+    insts = MEASURED_VALU.get((kernel, args.nx, args.nz, "fp32"))
+    if insts is None:
+        return None
+    achieved = insts / (ms * 1e-3)
+    return {"achieved": achieved, "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+            "frac": achieved / VALU_PEAK, "insts_per_launch": insts}
 
 
 def parse():
@@ -259,6 +282,7 @@ def main():
             "ms_per_launch": kern[dom],
             "basis": f"{dom_label}: {BASIS.get(dom_label, 'algorithmic bytes per launch')} "
                      "/ mean launch time from HIP events on the library stream",
+            "valu": valu_roofline(dom_label, args, kern[dom]),
         },
         "step_roofline": {
             "achieved": step_achieved,
