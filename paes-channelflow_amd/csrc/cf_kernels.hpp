@@ -43,7 +43,9 @@ struct ColShape {
   static constexpr int BYTES = 4 * NY * (int)sizeof(cx<T>);  // per column of the tile
   static constexpr int TK0 = 65536 / BYTES;
   static constexpr int TK = TK0 < MX ? TK0 : MX;             // columns per workgroup
-  static constexpr int NT = 256;
+  // Eight waves per tile: the workgroup's LDS (the padded tile) holds two
+  // workgroups per CU, so the wave count per workgroup sets the occupancy.
+  static constexpr int NT = 512;
 };
 
 template <typename T, int NX>
@@ -342,8 +344,9 @@ struct SynthArgs {
 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__(256) k_col_synth(SynthArgs<T> a) {
+__global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
+  static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
   const int f = blockIdx.z;
   const int z = blockIdx.y + 1;
@@ -1588,7 +1591,7 @@ k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
 
 // Small-grid closure, columns in: U, V, W band spectra -> u, v, w and uy, vy,
 // wy as x-spectral small rows (the y-inverse of a tile of TK columns and of
-// its i ky multiple in one batch).  grid (MX/TK, nz, 3 fields)
+// its i ky multiple in one batch).  grid (MX/TK, nz, 3 fields) x 512
 template <typename T>
 struct SgsColsInArgs {
   const cx<T>* U[3];      // U, V, W band spectra
@@ -1600,9 +1603,9 @@ struct SgsColsInArgs {
 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__(256) k_sgs_cols_in(SgsColsInArgs<T> a) {
+__global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 256, PL = padlen(NY);
+  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = padlen(NY);
   const int f = blockIdx.z, z = blockIdx.y + 1;
   if (f == 2 && z >= a.nz) return;                 // w lives on faces 1..nz-1
   const int k0 = blockIdx.x * TK, tid = threadIdx.x;
@@ -1634,7 +1637,7 @@ __global__ void __launch_bounds__(256) k_sgs_cols_in(SgsColsInArgs<T> a) {
 // Small-grid closure, columns out: the y-forward of the G and F^y rows of one
 // component, then R += G + i ky F^y on the band (Nyquist row untouched; for w
 // the (0,0) mode too, the plane-mean removal of comp1.f90:172).
-// grid (MX/TK, nz, 3 components)
+// grid (MX/TK, nz, 3 components) x 512
 template <typename T>
 struct SgsColsOutArgs {
   const cx<T>* O;         // G_u, Fy_u, G_v, Fy_v, G_w, Fy_w [6][slot][NY][MX]
@@ -1646,9 +1649,9 @@ struct SgsColsOutArgs {
 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__(256) k_sgs_cols_out(SgsColsOutArgs<T> a) {
+__global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 256, PL = padlen(NY);
+  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = padlen(NY);
   const int cmp = blockIdx.z, z = blockIdx.y + 1;
   if (cmp == 2 && z >= a.nz) return;               // rw(nnz) = 0
   const int k0 = blockIdx.x * TK, tid = threadIdx.x;
@@ -1699,8 +1702,9 @@ struct AnalArgs {
 // persistent loop over tiles with the next tile's columns held in registers
 // took 230-400 VGPRs, one wave per SIMD, and measured 37 % slower.)
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__(256) k_col_anal(AnalArgs<T> a) {
+__global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
+  static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
   constexpr int NKT = MX / TK;
   constexpr int LPT = 2 * TK * NYB / NT;       // tile elements per thread

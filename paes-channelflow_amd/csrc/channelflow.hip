@@ -463,7 +463,7 @@ struct EngineT : Engine {
       a.kyp = rptr(ctx->kyp);
       a.twNY = cptr(ctx->twNX);
       a.nz = nz;
-      hipLaunchKernelGGL((cf::k_sgs_cols_in<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((cf::k_sgs_cols_in<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(512), 0, s, a);
     }
     if (sgs) HIPCHK(hipMemsetAsync(ctx->sred, 0, 6 * S * sizeof(double), s));
     mark();
@@ -492,7 +492,7 @@ struct EngineT : Engine {
       b.kyp = rptr(ctx->kyp);
       b.twNY = cptr(ctx->twNX);
       b.nz = nz;
-      hipLaunchKernelGGL((cf::k_sgs_cols_out<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(256), 0, s, b);
+      hipLaunchKernelGGL((cf::k_sgs_cols_out<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(512), 0, s, b);
     }
     HIPCHK(hipGetLastError());
     return CF_OK;
