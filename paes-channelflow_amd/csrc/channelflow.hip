@@ -88,6 +88,9 @@ struct cf_ctx {
   void* sgsS = nullptr;         // small-grid closure: 6 x-spectral input rows (band-sized each)
   void* sgsO = nullptr;         // and its 6 output rows
   double* sred = nullptr;       // its SGS means [6][slots]
+  hipStream_t stream2 = nullptr;    // the small-grid closure's columns-in and rows run here,
+  hipEvent_t ev_fork = nullptr;     // concurrently with k_col_synth .. k_col_anal
+  hipEvent_t ev_join = nullptr;
   double* wpart = nullptr;      // isurface 2..5: wall-plane sums [2][NYB][8]
   void* X = nullptr;            // imodel 4: K at faces 0..nnz [nz+1][NYB][NXB]
   bool have_sred = false;       // sred holds the last mnout stage's means
@@ -262,6 +265,23 @@ struct EngineT : Engine {
     auto mark = [&]() {
       if (ev) (void)hipEventRecord(ev[ek++], s);
     };
+    // The small-grid closure's columns-in and row kernel depend only on the
+    // stage-start state and write only their own buffers: they run on the
+    // second stream beside k_wall .. k_col_anal (serially when per-kernel
+    // events are being recorded), and k_sgs_cols_out joins after k_col_anal.
+    bool small = false;
+    if constexpr (NX % 64 == 0) small = ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL;
+    const bool overlap = small && !ev;
+    if (overlap) {
+      HIPCHK(hipEventRecord(ctx->ev_fork, s));
+      HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+      if constexpr (NX % 64 == 0) {
+        int rc = sgs_cols_in(ctx, ctx->stream2);
+        if (rc == CF_OK) rc = sgs_rows(ctx, c, sgs, ctx->stream2);
+        if (rc != CF_OK) return rc;
+      }
+      HIPCHK(hipEventRecord(ctx->ev_join, ctx->stream2));
+    }
     mark();
     hipLaunchKernelGGL(cf::k_wall<T>, dim3(1), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
                        cptr(ctx->Ruo), cptr(ctx->Rvo), band_plane, c, ctx->ws);
@@ -340,17 +360,29 @@ struct EngineT : Engine {
       hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(ntiles), dim3(CS::NT), 0, s, a);
     }
     mark();
-    bool small = false;
     if constexpr (NX % 64 == 0) {
-      if (ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL) {
-        small = true;
-        int rc = sgs_small(ctx, c, sgs, mark);
+      if (small && !overlap) {
+        int rc = sgs_cols_in(ctx, s);
         if (rc != CF_OK) return rc;
       }
     }
-    if (!small) {
-      mark();
-      mark();
+    mark();
+    if constexpr (NX % 64 == 0) {
+      if (small) {
+        if (overlap) {
+          HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
+        } else {
+          int rc = sgs_rows(ctx, c, sgs, s);
+          if (rc != CF_OK) return rc;
+        }
+      }
+    }
+    mark();
+    if constexpr (NX % 64 == 0) {
+      if (small) {
+        int rc = sgs_small_out(ctx, s);
+        if (rc != CF_OK) return rc;
+      }
     }
     mark();
     {
@@ -443,34 +475,35 @@ struct EngineT : Engine {
   // The learned closure on the nnx x nny grid: y-inverse of u, v, w and their
   // y-derivatives to x-spectral small rows, the row kernel, then the
   // y-forward of its divergence rows added to the comp1 tendencies.
-  // mark() is called between the three sections (columns in, rows, columns out).
-  template <typename Mark>
-  int sgs_small(cf_ctx* ctx, const cf::Consts& c, bool sgs, Mark& mark) {
-    hipStream_t s = ctx->stream;
+  // The learned closure on the nnx x nny grid: sgs_cols_in (the y-inverse of
+  // U, V, W and their y-derivatives to x-spectral small rows), sgs_rows (the
+  // row kernel: network, TKE fallback, divergence rows) and sgs_small_out.
+  int sgs_cols_in(cf_ctx* ctx, hipStream_t s) {
+    constexpr int TK = CS::TK;
+    cf::SgsColsInArgs<T> a{};
+    a.U[0] = cptr(ctx->U);
+    a.U[1] = cptr(ctx->V);
+    a.U[2] = cptr(ctx->W);
+    a.S = cptr(ctx->sgsS);
+    a.sstride = (size_t)ctx->slots * NY * MX;
+    a.kyp = rptr(ctx->kyp);
+    a.twNY = cptr(ctx->twNX);
+    a.nz = ctx->nz;
+    hipLaunchKernelGGL((cf::k_sgs_cols_in<T, NX, NY>), dim3(MX / TK, ctx->nz, 3), dim3(512), 0, s, a);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+
+  int sgs_rows(cf_ctx* ctx, const cf::Consts& c, bool sgs, hipStream_t s) {
     const int nz = ctx->nz;
     const size_t S = ctx->slots;
     const size_t fstride = S * NY * MX;
-    constexpr int TK = CS::TK;
-    C* in = cptr(ctx->sgsS);
-    C* out = cptr(ctx->sgsO);
-    {
-      cf::SgsColsInArgs<T> a{};
-      a.U[0] = cptr(ctx->U);
-      a.U[1] = cptr(ctx->V);
-      a.U[2] = cptr(ctx->W);
-      a.S = in;
-      a.sstride = fstride;
-      a.kyp = rptr(ctx->kyp);
-      a.twNY = cptr(ctx->twNX);
-      a.nz = nz;
-      hipLaunchKernelGGL((cf::k_sgs_cols_in<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(512), 0, s, a);
-    }
     if (sgs) HIPCHK(hipMemsetAsync(ctx->sred, 0, 6 * S * sizeof(double), s));
-    mark();
     cf::SgsRowArgs<T> a{};
+    C* in = cptr(ctx->sgsS);
     a.S = in;
     a.sstride = fstride;
-    a.O = out;
+    a.O = cptr(ctx->sgsO);
     a.e = rptr(ctx->e);
     a.kxp = rptr(ctx->kxp);
     a.twNX = cptr(ctx->twNX);
@@ -481,19 +514,25 @@ struct EngineT : Engine {
     a.c.zchunk = ctx->zchunk_mlp;
     const int nch = (nz + ctx->zchunk_mlp - 1) / ctx->zchunk_mlp;
     hipLaunchKernelGGL((cf::k_sgs_rows<T, NX>), dim3(NY, nch), dim3(NX), 0, s, a, ctx->closure_net);
-    mark();
-    {
-      cf::SgsColsOutArgs<T> b{};
-      b.O = out;
-      b.ostride = fstride;
-      b.R[0] = cptr(ctx->Run);
-      b.R[1] = cptr(ctx->Rvn);
-      b.R[2] = cptr(ctx->Rwn);
-      b.kyp = rptr(ctx->kyp);
-      b.twNY = cptr(ctx->twNX);
-      b.nz = nz;
-      hipLaunchKernelGGL((cf::k_sgs_cols_out<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(512), 0, s, b);
-    }
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+
+  // Second part: the y-forward of the divergence rows added to the comp1
+  // tendencies (after k_col_anal has written them).
+  int sgs_small_out(cf_ctx* ctx, hipStream_t s) {
+    const int nz = ctx->nz;
+    constexpr int TK = CS::TK;
+    cf::SgsColsOutArgs<T> b{};
+    b.O = cptr(ctx->sgsO);
+    b.ostride = (size_t)ctx->slots * NY * MX;
+    b.R[0] = cptr(ctx->Run);
+    b.R[1] = cptr(ctx->Rvn);
+    b.R[2] = cptr(ctx->Rwn);
+    b.kyp = rptr(ctx->kyp);
+    b.twNY = cptr(ctx->twNX);
+    b.nz = nz;
+    hipLaunchKernelGGL((cf::k_sgs_cols_out<T, NX, NY>), dim3(MX / TK, nz, 3), dim3(512), 0, s, b);
     HIPCHK(hipGetLastError());
     return CF_OK;
   }
@@ -693,7 +732,11 @@ int cf_create(const cf_params* p, cf_ctx** out) {
 int cf_destroy(cf_ctx* ctx) {
   if (!ctx) return CF_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   for (void* p : ctx->allocs) (void)hipFree(p);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx->eng;
   delete ctx;
@@ -1410,6 +1453,11 @@ int cf_set_closure_grid(cf_ctx* ctx, int32_t grid) {
   }
   HIPCHK(hipSetDevice(ctx->p.device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (grid == CF_CLOSURE_GRID_SMALL && !ctx->stream2) {
+    HIPCHK(hipStreamCreate(&ctx->stream2));                   // default priority
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  }
   if (grid == CF_CLOSURE_GRID_SMALL && !ctx->sgsS) {
     const size_t elem = ctx->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
     const size_t band = (size_t)ctx->slots * ctx->ny * ctx->mx * elem;
