@@ -35,9 +35,8 @@ extern "C" {
 #define CF_E_UTAU (-3)        /* utau > 10 wall-model guard: lowerbc.f90:77-80,    */
                               /* upperbc.f90:253-256 (the reference calls lesabort) */
 #define CF_E_NONFINITE (-4)   /* a field became NaN/Inf                           */
-#define CF_E_ECLAMP (-5)      /* the 3/2-grid SGS TKE fell below emin on some      */
-                              /* level; that clamp (comp1.f90:109,121) is not      */
-                              /* supported on the device path                      */
+#define CF_E_ECLAMP (-5)      /* reserved: no longer returned (the comp1 clamp of  */
+                              /* the expanded e, comp1.f90:109,121, is carried out) */
 #define CF_E_NODEV (-6)       /* no HIP device                                     */
 
 #define CF_PREC_FP64 0        /* every field and transform in FP64                  */

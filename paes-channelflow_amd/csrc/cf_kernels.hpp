@@ -437,8 +437,16 @@ struct RowArgs {
   size_t pstride;          // slots*NYB
   int* flags;
   const T* X;              // imodel 4: K at faces 0..nnz [face][NYB][NXB] (k_shear)
+  int* dirty;              // [f] set where the 3/2-grid e of face f fell below emin; [0]: any
   Consts c;
 };
+
+// The expanded e of face f fell below emin at this point (comp1.f90:109,121):
+// the flux corrections of k_eclamp_* run for the face.
+__device__ __forceinline__ void mark_clamp(int* dirty, int f) {
+  atomicOr(dirty + f, 1);
+  atomicOr(dirty, 1);
+}
 
 // Row transforms of length NXB: one wavefront owns whole transforms (lanes
 // exchange through LDS with wave-level ordering) when NXB is a multiple of
@@ -588,6 +596,178 @@ __global__ void __launch_bounds__(3 * NX / 2) k_wall_plane(const cx<T>* __restri
   if (tid < 8) part[((size_t)wall * NYB + y) * 8 + tid] = q[0];
 }
 
+// ---------------------------------------------------------------------------
+// The comp1 clamp of the expanded SGS TKE (comp1.f90:109,121).  Where e on
+// the 3/2 grid falls below emin the reference carries on with max(e, emin):
+// its K, advection and dissipation use the clamped values (so do the row
+// kernels), and ex, ey are the 3/2-grid derivatives of the clamped field
+// (derivs2.f90:20-21).  The row kernels differentiate the band-limited e, so
+// for every marked face f the derivatives of the clamp residual
+// d = max(e, emin) - e are added to the e-fluxes of the levels that read
+// ex, ey of face f (level f; also nnz-1 for f = nnz-2, derivs2.f90:279):
+//   F_x^e += 2 K dx(d),  F_y^e += 2 K dy(d)   (compsgs3.f90:185-200),
+// with K at the level's face.  dx, dy are xderiv/yderiv on the 3/2 grid
+// (k = 0 and the Nyquist mode dropped, filter_fftw.f90:138-248).
+//   k_eclamp_rows  d and dx(d)                  grid NYB, one thread per point
+//   k_eclamp_cols  dy(d), in place of d         grid NXB/16, 256 threads
+//   k_eclamp_flux  the corrections, x-forward, added to the G_e and F_y^e
+//                  rows of the analysis buffer   grid NYB
+// Each pass returns at once when no face is marked.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct EclampArgs {
+  const cx<T>* B;          // synthesis rows (e: field 6)
+  size_t bstride;
+  cx<T>* A;                // analysis rows (G_e: field 6, F_y^e: field 7)
+  size_t astride;
+  const int* dirty;
+  T* DX;                   // dx(d) [face][NYB][NXB]
+  T* DY;                   // d, then dy(d)
+  const T* X;              // imodel 4: K at the faces
+  const T* kxp;
+  const cx<T>* twNXB;
+  double kxb, kyb;         // 2 pi / xl, 2 pi / yl
+  Consts c;
+};
+
+// Wavenumber of FFT index k of length n (k = 0 and n/2 dropped).
+template <typename T>
+__device__ __forceinline__ T deriv_k(int k, int n, double unit) {
+  if (k > 0 && k < n / 2) return (T)(unit * k);
+  if (k > n / 2) return (T)(unit * (k - n));
+  return T(0);
+}
+
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_eclamp_rows(EclampArgs<T> a) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT, NYB = NXB;
+  if (!a.dirty[0]) return;
+  __shared__ cx<T> buf[padlen(NXB)];
+  __shared__ cx<T> tw[NXB];
+  const int tid = threadIdx.x, y = blockIdx.x, nz = a.c.nz;
+  const T emin = (T)a.c.emin;
+  const size_t pp = (size_t)NYB * NXB;
+  load_twiddles(tw, a.twNXB, NXB, tid, NT);
+  for (int f = 1; f <= nz - 2; ++f) {
+    if (!a.dirty[f]) continue;
+    __syncthreads();
+    const cx<T>* src = a.B + 6 * a.bstride + (size_t)f * NYB * MX + (size_t)y * MX;
+    fill_pair<T, NX>(buf, src, false, nullptr, false, a.kxp, tid, NT);
+    __syncthreads();
+    fft_run<T, NXB, true, 1, NT>(buf, tw, tid);
+    const T e = buf[pidx(tid)].x;
+    const T d = (e < emin ? emin : e) - e;
+    __syncthreads();
+    buf[pidx(tid)] = cx<T>{d, T(0)};
+    __syncthreads();
+    fft_run<T, NXB, false, 1, NT>(buf, tw, tid);
+    {
+      const T kk = deriv_k<T>(tid, NXB, a.kxb);
+      const cx<T> z = buf[pidx(tid)];
+      buf[pidx(tid)] = cx<T>{-kk * z.y, kk * z.x};
+    }
+    __syncthreads();
+    fft_run<T, NXB, true, 1, NT>(buf, tw, tid);
+    const size_t o = (size_t)f * pp + (size_t)y * NXB + tid;
+    a.DX[o] = buf[pidx(tid)].x / T(NXB);
+    a.DY[o] = d;
+  }
+}
+
+template <typename T, int NX>
+__global__ void __launch_bounds__(256) k_eclamp_cols(EclampArgs<T> a) {
+  constexpr int NXB = 3 * NX / 2, NYB = NXB, P = 8, NT = 256, PL = padlen(NYB);
+  static_assert(NXB % (2 * P) == 0, "column pairs per workgroup");
+  if (!a.dirty[0]) return;
+  __shared__ cx<T> buf[P * PL];
+  __shared__ cx<T> tw[NYB];
+  const int tid = threadIdx.x, x0 = blockIdx.x * 2 * P, nz = a.c.nz;
+  const size_t pp = (size_t)NYB * NXB;
+  load_twiddles(tw, a.twNXB, NYB, tid, NT);                  // NYB == NXB
+  for (int f = 1; f <= nz - 2; ++f) {
+    if (!a.dirty[f]) continue;
+    T* D = a.DY + (size_t)f * pp;
+    __syncthreads();
+    for (int i = tid; i < P * NYB; i += NT) {                  // columns x0+2q, x0+2q+1 as one complex
+      const int q = i % P, yy = i / P;
+      const T* r = D + (size_t)yy * NXB + x0 + 2 * q;
+      buf[q * PL + pidx(yy)] = cx<T>{r[0], r[1]};
+    }
+    __syncthreads();
+    fft_waves<T, NYB, false, P, NT / 64>(buf, tw, tid);
+    __syncthreads();
+    for (int i = tid; i < P * NYB; i += NT) {
+      const int q = i / NYB, k = i % NYB;
+      const T kk = deriv_k<T>(k, NYB, a.kyb);
+      const cx<T> z = buf[q * PL + pidx(k)];
+      buf[q * PL + pidx(k)] = cx<T>{-kk * z.y, kk * z.x};
+    }
+    __syncthreads();
+    fft_waves<T, NYB, true, P, NT / 64>(buf, tw, tid);
+    __syncthreads();
+    const T s = T(1) / T(NYB);
+    for (int i = tid; i < P * NYB; i += NT) {
+      const int q = i % P, yy = i / P;
+      const cx<T> v = buf[q * PL + pidx(yy)];
+      T* r = D + (size_t)yy * NXB + x0 + 2 * q;
+      r[0] = v.x * s;
+      r[1] = v.y * s;
+    }
+  }
+}
+
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 2) k_eclamp_flux(EclampArgs<T> a) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT, NYB = NXB;
+  if (!a.dirty[0]) return;
+  __shared__ cx<T> buf[padlen(NXB)];
+  __shared__ cx<T> tw[NXB];
+  const int tid = threadIdx.x, y = blockIdx.x, nz = a.c.nz;
+  const T emin = (T)a.c.emin, alk = (T)a.c.alk, xkmax = (T)a.c.xkmax;
+  const size_t pp = (size_t)NYB * NXB;
+  const size_t plane = (size_t)NYB * MX;
+  load_twiddles(tw, a.twNXB, NXB, tid, NT);
+  for (int iz = 1; iz <= nz - 1; ++iz) {
+    const int ie = (iz == nz - 1) ? nz - 2 : iz;
+    if (ie < 1 || !a.dirty[ie]) continue;
+    __syncthreads();
+    T K;
+    if (a.c.imodel == 4) {
+      K = a.X[(size_t)iz * pp + (size_t)y * NXB + tid];
+    } else {                                                    // complen.f90:88-89 on the clamped e
+      const cx<T>* src = a.B + 6 * a.bstride + (size_t)iz * plane + (size_t)y * MX;
+      fill_pair<T, NX>(buf, src, false, nullptr, false, a.kxp, tid, NT);
+      __syncthreads();
+      fft_run<T, NXB, true, 1, NT>(buf, tw, tid);
+      T e = buf[pidx(tid)].x;
+      e = e > emin ? e : emin;
+      K = T(0.1) * alk * sqrt(e);
+      K = K < xkmax ? K : xkmax;
+      __syncthreads();
+    }
+    const size_t o = (size_t)ie * pp + (size_t)y * NXB + tid;
+    buf[pidx(tid)] = cx<T>{T(2) * K * a.DX[o], T(2) * K * a.DY[o]};
+    __syncthreads();
+    fft_run<T, NXB, false, 1, NT>(buf, tw, tid);
+    if (tid < MX) {                                             // as anal_pairs
+      const T s = T(0.5) / T(NXB);
+      const int k = tid;
+      const cx<T> zk = buf[pidx(k)];
+      const cx<T> zn = buf[pidx(k == 0 ? 0 : NXB - k)];
+      const cx<T> fx = {(zk.x + zn.x) * s, (zk.y - zn.y) * s};
+      const cx<T> fy = {(zk.y + zn.y) * s, (zn.x - zk.x) * s};
+      const T kx = a.kxp[k];
+      cx<T>* ge = a.A + 6 * a.astride + (size_t)iz * plane + (size_t)y * MX + k;
+      cx<T>* fe = a.A + 7 * a.astride + (size_t)iz * plane + (size_t)y * MX + k;
+      const cx<T> g = *ge, h = *fe;
+      *ge = cx<T>{g.x - kx * fx.y, g.y + kx * fx.x};
+      *fe = cx<T>{h.x + fy.x, h.y + fy.y};
+    }
+  }
+}
+
 // imodel 4 (complen.f90:13-17,22-23): the eddy diffusivity at faces 0..nnz
 // from the face shear
 //   S(f) = G(f) + G(f+1) + (uz(f) + wx(f))^2 + (vz(f) + wy(f))^2,
@@ -687,7 +867,6 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
   const bool mn = c.mnout != 0;
   const bool sgs_mom = c.sgs_momentum != 0;
-  int clampflag = 0;
 
   auto xkk_of = [&](T e) {
     T v = T(0.1) * alk * sqrt(e);
@@ -718,7 +897,8 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     um = ra[0]; vm = rb[0];
     u0 = ra[1]; v0 = rb[1];
     wm = ra[2]; w0 = ra[3];
-    if (rb[2] < emin || rb[3] < emin) clampflag = 1;
+    if (rb[2] < emin) mark_clamp(a.dirty, clampF(L0));
+    if (rb[3] < emin) mark_clamp(a.dirty, clampF(L1));
     em = rb[2] > emin ? rb[2] : emin;
     e0 = rb[3] > emin ? rb[3] : emin;
     wxm = ra[4]; wym = rb[4];
@@ -758,7 +938,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
       synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
       up = ra[0]; vp = rb[0];
       wp = ra[1];
-      if (rb[1] < emin) clampflag = 1;
+      if (rb[1] < emin) mark_clamp(a.dirty, clampF(iz + 1));
       ep = rb[1] > emin ? rb[1] : emin;
       ux = ra[2]; uy = rb[2];
       vx = ra[3]; vy = rb[3];
@@ -891,7 +1071,6 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     em = e0; e0 = ep;
     wxm = wx0; wym = wy0;
   }
-  if (clampflag) atomicOr(a.flags, FLAG_ECLAMP);
 }
 
 // ---------------------------------------------------------------------------
@@ -951,7 +1130,6 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
   const float dslf = (float)c.dsl, dzf = (float)c.dz, zlf = (float)c.zl;
   const bool mn = c.mnout != 0;
-  int clampflag = 0;
   unsigned long long nacc = 0;
 
   auto xkk_of = [&](T e) {
@@ -993,7 +1171,8 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
     u0 = ra[1]; v0 = rb[1];
     up = ra[2]; vp = rb[2];
     wm = ra[3]; w0 = ra[4];
-    if (rb[3] < emin || rb[4] < emin) clampflag = 1;
+    if (rb[3] < emin) mark_clamp(a.dirty, clampF(zs - 1));
+    if (rb[4] < emin) mark_clamp(a.dirty, clampF(zs));
     em = rb[3] > emin ? rb[3] : emin;
     e0 = rb[4] > emin ? rb[4] : emin;
     wxm = ra[5]; wym = rb[5];
@@ -1034,7 +1213,7 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
       synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
       upp = ra[0]; vpp = rb[0];
       wp = ra[1];
-      if (rb[1] < emin) clampflag = 1;
+      if (rb[1] < emin) mark_clamp(a.dirty, clampF(iz + 1));
       ep = rb[1] > emin ? rb[1] : emin;
       ux1 = ra[2]; uy1 = rb[2];
       vx1 = ra[3]; vy1 = rb[3];
@@ -1169,7 +1348,6 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
     ok0 = ok1;
     fuw = tuw0; fvw = tvw0;
   }
-  if (clampflag) atomicOr(a.flags, FLAG_ECLAMP);
   if (lane == 0) {
     atomicAdd(counts, nacc);
     atomicAdd(counts + 1, 64ull * (unsigned long long)(z1 - z0));

@@ -92,6 +92,9 @@ struct cf_ctx {
   hipEvent_t ev_fork = nullptr;     // concurrently with k_col_synth .. k_col_anal
   hipEvent_t ev_join = nullptr;
   double* wpart = nullptr;      // isurface 2..5: wall-plane sums [2][NYB][8]
+  int* dirty = nullptr;         // faces whose expanded e was clamped this stage; [0]: any
+  void* eclDX = nullptr;        // clamp residual derivatives [nz+1][NYB][NXB]
+  void* eclDY = nullptr;
   void* X = nullptr;            // imodel 4: K at faces 0..nnz [nz+1][NYB][NXB]
   bool have_sred = false;       // sred holds the last mnout stage's means
   // box-filter work buffers (allocated on first use)
@@ -197,6 +200,9 @@ struct EngineT : Engine {
     if ((rc = dalloc(ctx, &ctx->ws, (size_t)1)) != CF_OK) return rc;
     if (ctx->p.isurface >= 2 && ctx->p.isurface <= 5)      // wall-plane statistics
       if ((rc = dalloc(ctx, &ctx->wpart, (size_t)2 * NYB * 8)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, &ctx->dirty, (size_t)ctx->nz + 3)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, (T**)&ctx->eclDX, (size_t)(ctx->nz + 1) * NYB * NXB)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, (T**)&ctx->eclDY, (size_t)(ctx->nz + 1) * NYB * NXB)) != CF_OK) return rc;
     if (ctx->p.imodel == 4)                                 // K at faces 0..nnz on the 3/2 grid
       if ((rc = dalloc(ctx, (T**)&ctx->X, (size_t)(ctx->nz + 1) * NYB * NXB)) != CF_OK) return rc;
     // tables
@@ -282,6 +288,7 @@ struct EngineT : Engine {
       }
       HIPCHK(hipEventRecord(ctx->ev_join, ctx->stream2));
     }
+    HIPCHK(hipMemsetAsync(ctx->dirty, 0, ((size_t)nz + 3) * sizeof(int), s));
     mark();
     hipLaunchKernelGGL(cf::k_wall<T>, dim3(1), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
                        cptr(ctx->Ruo), cptr(ctx->Rvo), band_plane, c, ctx->ws);
@@ -328,6 +335,7 @@ struct EngineT : Engine {
       a.pstride = (size_t)ctx->slots * NYB;
       a.flags = &ctx->ws->flags;
       a.X = rptr(ctx->X);
+      a.dirty = ctx->dirty;
       a.c = c;
       bool learned = false;
       if constexpr ((3 * NX / 2) % 64 == 0) {
@@ -342,6 +350,24 @@ struct EngineT : Engine {
       }
       const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
       if (!learned) hipLaunchKernelGGL((cf::k_rows<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+      // the comp1 clamp of the expanded e, where it fired (no work otherwise)
+      cf::EclampArgs<T> ea{};
+      ea.B = cptr(ctx->Bs);
+      ea.bstride = bstride;
+      ea.A = cptr(ctx->Ba);
+      ea.astride = bstride;
+      ea.dirty = ctx->dirty;
+      ea.DX = rptr(ctx->eclDX);
+      ea.DY = rptr(ctx->eclDY);
+      ea.X = rptr(ctx->X);
+      ea.kxp = rptr(ctx->kxp);
+      ea.twNXB = cptr(ctx->twNXB);
+      ea.kxb = 2.0 * PI / ctx->p.xl;
+      ea.kyb = 2.0 * PI / ctx->p.yl;
+      ea.c = c;
+      hipLaunchKernelGGL((cf::k_eclamp_rows<T, NX>), dim3(NYB), dim3(RS::NT), 0, s, ea);
+      hipLaunchKernelGGL((cf::k_eclamp_cols<T, NX>), dim3(NXB / 16), dim3(256), 0, s, ea);
+      hipLaunchKernelGGL((cf::k_eclamp_flux<T, NX>), dim3(NYB), dim3(RS::NT), 0, s, ea);
     }
     mark();
     {
@@ -652,11 +678,6 @@ int check_flags(cf_ctx* ctx) {
   if (flags & cf::FLAG_NONFINITE) {
     ctx->err = "non-finite SGS TKE";
     return CF_E_NONFINITE;
-  }
-  if (flags & cf::FLAG_ECLAMP) {
-    ctx->err = "3/2-grid SGS TKE below emin: the comp1 clamp (comp1.f90:109,121) is not "
-               "supported on the device path";
-    return CF_E_ECLAMP;
   }
   return CF_OK;
 }
