@@ -18,6 +18,9 @@
   filtered fine run's 0.012).  The seeded noise therefore evolves differently
   near the walls, and the fluctuation fields decorrelate within 200 steps.
 * The same with the network on the nnx x nny grid (CF_CLOSURE_GRID_SMALL).
+* FP32 against FP64 after 2000 steps at 256^2 x 128 (the metric
+  configuration's comparison): mean profile and resolved TKE within 1e-3,
+  x-y spectra within 1e-2 up to kmax/4 and 2e-2 over all k.
 * 32^3 for 100 steps (the CPU-reference case): FP64 fields within 1e-9 of
   the oracle.
 """
@@ -113,6 +116,39 @@ def test_learned_small_grid_against_fp64_tke_256():
     e = _report("fp32+learned (small grid) vs fp64 TKE after 200 steps:", sl, s64, ml, m64, 128)
     assert e["uxym"] < 1e-2, e
     assert abs(e["englez_ratio"] - 1.0) < 5e-2, e
+
+
+def test_fp32_path_after_2000_steps_256():
+    """The metric configuration's own comparison: 2000 steps at 256^2 x 128
+    (t = 2.6 at CFL 0.3), FP32 against FP64, both with the TKE closure.
+    Measured: mean profile 1e-4, resolved TKE 8e-5 (rel L2), x-y spectra at
+    the reference's levels 4.4e-3 up to kmax/4 and 6.1e-3 over all k."""
+    import channelflow as C
+    z0 = C.z0_of_retau(395.0)
+    dt = 0.3 / (3.2 * (3 * 256 / 2) / (2 * math.pi) + 0.5 / (2.0 / 128))
+    zl = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 18, 24, 30, 36, 48, 64]
+    out = {}
+    for prec in (C.CF_PREC_FP64, C.CF_PREC_FP32):
+        g = C.ChannelFlow(C.RunParams(256, 256, 128, z0=z0, dt=dt, precision=prec))
+        g.seeded_init(10972, 0.1)
+        for _ in range(4):
+            g.step(500)
+            g.sync()
+            print("precision", prec, "utau", g.scalars()["utau"], flush=True)
+        h, _, _ = g.calc_spectra(zl)
+        out[prec] = (np.asarray(h), g.compmn())
+        g.close()
+    (h32, m32), (h64, m64) = out[C.CF_PREC_FP32], out[C.CF_PREC_FP64]
+    r = O.MEAN_NAMES.index
+    du = float(np.max(np.abs(m32[r("uxym")] - m64[r("uxym")]) / np.abs(m64[r("uxym")])))
+    de = rel(m32[r("englez")], m64[r("englez")])
+    kq = (h64.shape[1] - 1) // 4
+    ds_q = float(np.max(np.abs(h32[:, 1:kq + 1] - h64[:, 1:kq + 1]) / np.abs(h64[:, 1:kq + 1])))
+    ds = float(np.max(np.abs(h32[:, 1:] - h64[:, 1:]) / np.abs(h64[:, 1:])))
+    print(f"fp32 vs fp64 after 2000 steps: uxym {du:.2e} englez {de:.2e} spectra k<=kmax/4 {ds_q:.2e} all {ds:.2e}",
+          flush=True)
+    assert du < 1e-3 and de < 1e-3, (du, de)
+    assert ds_q < 1e-2 and ds < 2e-2, (ds_q, ds)
 
 
 def test_fp64_100_steps_32cubed_matches_oracle():
