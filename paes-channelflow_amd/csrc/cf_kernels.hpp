@@ -337,6 +337,7 @@ struct SynthArgs {
   const cx<T>* Ex;        // e, x-spectral small grid (rows of physical y)
   cx<T>* B;               // 8 outputs: u, uy, v, vy, w, wy, e, ey ; stride bstride
   size_t bstride;         // elements per output field (all slots)
+  int lean;               // 1: uy, vy, wy are not needed (lean row kernel)
   const T* kyp;           // ky of band row j (Nyquist row -> 0)
   const cx<T>* twNY;
   const cx<T>* twNYB;
@@ -386,6 +387,24 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
       v = (j == NY / 2) ? cx<T>{0, 0} : cscale(v, s);
     }
     __syncthreads();
+  }
+  if (a.lean && f < 3) {                 // the value only
+    for (int i = tid; i < TK * NYB; i += NT) {
+      const int c = i / NYB, n = i - c * NYB;
+      int j = -1;
+      if (n < NY / 2) j = n;
+      else if (n > NYB - NY / 2) j = n - NYB + NY;
+      buf[c * padlen(NYB) + pidx(n)] = j >= 0 ? spec[c * padlen(NY) + pidx(j)] : cx<T>{0, 0};
+    }
+    __syncthreads();
+    fft_waves<T, NYB, true, TK, NT / 64>(buf, twb, tid);
+    __syncthreads();
+    cx<T>* o0 = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
+    for (int i = tid; i < TK * NYB; i += NT) {
+      const int c = i % TK, n = i / TK;
+      o0[(size_t)n * MX + c] = buf[c * padlen(NYB) + pidx(n)];
+    }
+    return;
   }
   // zero-pad the band into the 3/2 grid; second half of the batch holds i ky S
 #pragma unroll
@@ -837,7 +856,11 @@ __global__ void __launch_bounds__(3 * NX / 2) k_shear(const cx<T>* __restrict__ 
   }
 }
 
-template <typename T, int NX>
+// LEAN: the learned closure on the nnx x nny grid supplies the interior SGS
+// momentum stresses (Consts::sgs_momentum == 0) and k_divz_band the
+// divergence, so the velocity gradients are not synthesised: three pairs per
+// level instead of six (u, v; w, e; ex, ey).
+template <typename T, int NX, bool LEAN = false>
 __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   using F = RowFFT<T, NX>;
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
@@ -866,7 +889,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
   const T ugal = (T)c.ugal, vgal = (T)c.vgal;
   const bool mn = c.mnout != 0;
-  const bool sgs_mom = c.sgs_momentum != 0;
+  const bool sgs_mom = !LEAN && c.sgs_momentum != 0;
 
   auto xkk_of = [&](T e) {
     T v = T(0.1) * alk * sqrt(e);
@@ -928,7 +951,23 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
   };
   for (int iz = z0; iz < z1; ++iz) {
     T ux, uy, vx, vy, wx0, wy0, ex, ey;
-    {
+    if constexpr (LEAN) {
+      T ra[3], rb[3];
+      const cx<T>* sa[6];
+      const cx<T>* sb[6];
+      const bool da[3] = {false, false, true};
+      const bool db[3] = {false, false, false};
+      level_srcs(iz, sa, sb);
+      sa[2] = sa[5];
+      sb[2] = sb[5];
+      synth_pairs<T, NX, 3>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      up = ra[0]; vp = rb[0];
+      wp = ra[1];
+      if (rb[1] < emin) mark_clamp(a.dirty, clampF(iz + 1));
+      ep = rb[1] > emin ? rb[1] : emin;
+      ux = uy = vx = vy = wx0 = wy0 = T(0);
+      ex = ra[2]; ey = rb[2];
+    } else {
       T ra[6], rb[6];
       const cx<T>* sa[6];
       const cx<T>* sb[6];
@@ -1012,7 +1051,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
       fa[4] = Fxe; fb[4] = He;
       fa[5] = Fye; fb[5] = T(0);
       const double dv = (double)(ux + vy + wz0);                  // comp1.f90:145
-      sums[0] = dv * dv;
+      sums[0] = LEAN ? 0.0 : dv * dv;
       sums[1] = (double)tuu; sums[2] = (double)tvv; sums[3] = (double)tuv;
       sums[4] = (double)tww0; sums[5] = (double)tuw0; sums[6] = (double)tvw0;
     }
@@ -2335,11 +2374,11 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
 
 // sums of the per-row partials: divz (comp1.f90:145) and the SGS means
 // (compsgs3.f90:236-243).  grid (slots), block 256
-__global__ void k_reduce_part(const double* __restrict__ part, size_t pstride, int nyb, int nq,
+__global__ void k_reduce_part(const double* __restrict__ part, size_t pstride, int nyb, int q0, int nq,
                               double scale, double* out, int slots) {
   __shared__ double red[256];
   const int L = blockIdx.x;
-  for (int q = 0; q < nq; ++q) {
+  for (int q = q0; q < nq; ++q) {
     double s = 0;
     for (int y = threadIdx.x; y < nyb; y += 256) s += part[(size_t)q * pstride + (size_t)L * nyb + y];
     red[threadIdx.x] = s;
@@ -2351,6 +2390,43 @@ __global__ void k_reduce_part(const double* __restrict__ part, size_t pstride, i
     if (threadIdx.x == 0) out[(size_t)q * slots + L] = red[0] * scale;
     __syncthreads();
   }
+}
+
+// divz (comp1.f90:145) from the band spectra: the 3/2-grid plane mean of
+// (ux + vy + wz)^2 is, by Parseval, the weighted power of
+// i kx U + i ky V + (W(iz) - W(iz-1))/dz (weight 2 for kx > 0), with
+// w = 0 at faces 0 and nnz.  grid (nz, parts): each workgroup sums a slice
+// of the plane into out[iz] (cleared beforehand) with an atomic add.
+template <typename T>
+__global__ void __launch_bounds__(256) k_divz_band(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
+                                                   const cx<T>* __restrict__ W, const T* __restrict__ kxp,
+                                                   const T* __restrict__ kyp, int mx, int ny, int nz,
+                                                   double dzm, double* out) {
+  __shared__ double red[256];
+  const int iz = blockIdx.x + 1, tid = threadIdx.x;
+  const size_t plane = (size_t)mx * ny;
+  const cx<T>* u = U + (size_t)iz * plane;
+  const cx<T>* v = V + (size_t)iz * plane;
+  const cx<T>* w = W + (size_t)iz * plane;
+  const cx<T>* wm = W + (size_t)(iz - 1) * plane;
+  double s = 0.0;
+  for (size_t m = (size_t)blockIdx.y * 256 + tid; m < plane; m += (size_t)gridDim.y * 256) {
+    const int k = (int)(m % mx), j = (int)(m / mx);
+    const double kx = (double)kxp[k], ky = (double)kyp[j];
+    double wr = 0, wi = 0, mr = 0, mi = 0;
+    if (iz < nz) { wr = (double)w[m].x; wi = (double)w[m].y; }
+    if (iz > 1) { mr = (double)wm[m].x; mi = (double)wm[m].y; }
+    const double dr = -kx * (double)u[m].y - ky * (double)v[m].y + (wr - mr) * dzm;
+    const double di = kx * (double)u[m].x + ky * (double)v[m].x + (wi - mi) * dzm;
+    s += (k == 0 ? 1.0 : 2.0) * (dr * dr + di * di);
+  }
+  red[tid] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  if (tid == 0) atomicAdd(out + iz, red[0]);
 }
 
 // i kx (dir 0) or i ky (dir 1) times a band spectrum, slots 1..nz.

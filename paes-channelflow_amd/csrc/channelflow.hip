@@ -271,23 +271,13 @@ struct EngineT : Engine {
     auto mark = [&]() {
       if (ev) (void)hipEventRecord(ev[ek++], s);
     };
-    // The small-grid closure's columns-in and row kernel depend only on the
-    // stage-start state and write only their own buffers: they run on the
-    // second stream beside k_wall .. k_col_anal (serially when per-kernel
+    // The small-grid closure's divz, columns-in and row kernel depend only on
+    // the stage-start state and write only their own buffers: they run on the
+    // second stream beside k_rows .. k_col_anal (serially when per-kernel
     // events are being recorded), and k_sgs_cols_out joins after k_col_anal.
     bool small = false;
     if constexpr (NX % 64 == 0) small = ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL;
     const bool overlap = small && !ev;
-    if (overlap) {
-      HIPCHK(hipEventRecord(ctx->ev_fork, s));
-      HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-      if constexpr (NX % 64 == 0) {
-        int rc = sgs_cols_in(ctx, ctx->stream2);
-        if (rc == CF_OK) rc = sgs_rows(ctx, c, sgs, ctx->stream2);
-        if (rc != CF_OK) return rc;
-      }
-      HIPCHK(hipEventRecord(ctx->ev_join, ctx->stream2));
-    }
     HIPCHK(hipMemsetAsync(ctx->dirty, 0, ((size_t)nz + 3) * sizeof(int), s));
     mark();
     hipLaunchKernelGGL(cf::k_wall<T>, dim3(1), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
@@ -305,8 +295,20 @@ struct EngineT : Engine {
       a.twNY = cptr(ctx->twNX);
       a.twNYB = cptr(ctx->twNXB);
       a.nz = nz;
+      a.lean = small ? 1 : 0;
       hipLaunchKernelGGL((cf::k_col_synth<T, NX, NY>), dim3(MX / CS::TK, nz, 4), dim3(CS::NT), 0,
                          s, a);
+    }
+    if (overlap) {                 // forked after k_col_synth: the closure chain overlaps k_rows
+      HIPCHK(hipEventRecord(ctx->ev_fork, s));
+      HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+      if constexpr (NX % 64 == 0) {
+        int rc = divz_band(ctx, c.dzm, ctx->stream2);
+        if (rc == CF_OK) rc = sgs_cols_in(ctx, ctx->stream2);
+        if (rc == CF_OK) rc = sgs_rows(ctx, c, sgs, ctx->stream2);
+        if (rc != CF_OK) return rc;
+      }
+      HIPCHK(hipEventRecord(ctx->ev_join, ctx->stream2));
     }
     if (ctx->p.isurface >= 2 && ctx->p.isurface <= 5) {    // wall-plane statistics, then the wall coefficients
       hipLaunchKernelGGL((cf::k_wall_plane<T, NX>), dim3(NYB, 2), dim3(RS::NT), 0, s, cptr(ctx->Bs), bstride,
@@ -349,7 +351,12 @@ struct EngineT : Engine {
         }
       }
       const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
-      if (!learned) hipLaunchKernelGGL((cf::k_rows<T, NX>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+      if (!learned) {
+        if (small)
+          hipLaunchKernelGGL((cf::k_rows<T, NX, true>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+        else
+          hipLaunchKernelGGL((cf::k_rows<T, NX, false>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+      }
       // the comp1 clamp of the expanded e, where it fired (no work otherwise)
       cf::EclampArgs<T> ea{};
       ea.B = cptr(ctx->Bs);
@@ -388,7 +395,8 @@ struct EngineT : Engine {
     mark();
     if constexpr (NX % 64 == 0) {
       if (small && !overlap) {
-        int rc = sgs_cols_in(ctx, s);
+        int rc = divz_band(ctx, c.dzm, s);
+        if (rc == CF_OK) rc = sgs_cols_in(ctx, s);
         if (rc != CF_OK) return rc;
       }
     }
@@ -457,8 +465,12 @@ struct EngineT : Engine {
       hipLaunchKernelGGL((cf::k_erows<T, NX>), dim3(NY / 2, nz), dim3(MX), 0, s, a);
     }
     mark();
-    hipLaunchKernelGGL(cf::k_reduce_part, dim3(ctx->slots), dim3(256), 0, s, ctx->part,
-                       (size_t)ctx->slots * NYB, NYB, sgs ? 7 : 1, c.xymb, ctx->red, ctx->slots);
+    {
+      const int q0 = small ? 1 : 0, nq = sgs ? 7 : 1;        // divz from k_divz_band when small
+      if (nq > q0)
+        hipLaunchKernelGGL(cf::k_reduce_part, dim3(ctx->slots), dim3(256), 0, s, ctx->part,
+                           (size_t)ctx->slots * NYB, NYB, q0, nq, c.xymb, ctx->red, ctx->slots);
+    }
     mark();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -504,6 +516,17 @@ struct EngineT : Engine {
   // The learned closure on the nnx x nny grid: sgs_cols_in (the y-inverse of
   // U, V, W and their y-derivatives to x-spectral small rows), sgs_rows (the
   // row kernel: network, TKE fallback, divergence rows) and sgs_small_out.
+  // divz from the stage-start band spectra (the lean row kernel does not
+  // synthesise the velocity gradients); into the q = 0 slots of red.
+  int divz_band(cf_ctx* ctx, double dzm, hipStream_t s) {
+    HIPCHK(hipMemsetAsync(ctx->red, 0, (size_t)ctx->slots * sizeof(double), s));   // the q = 0 slots
+    const int parts = std::max(1, std::min(16, (MX * NY) / 2048));
+    hipLaunchKernelGGL(cf::k_divz_band<T>, dim3(ctx->nz, parts), dim3(256), 0, s, cptr(ctx->U), cptr(ctx->V),
+                       cptr(ctx->W), rptr(ctx->kxp), rptr(ctx->kyp), MX, NY, ctx->nz, dzm, ctx->red);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+
   int sgs_cols_in(cf_ctx* ctx, hipStream_t s) {
     constexpr int TK = CS::TK;
     cf::SgsColsInArgs<T> a{};

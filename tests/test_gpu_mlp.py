@@ -211,6 +211,8 @@ def test_small_grid_closure_matches_oracle(kind, nz, stages):
         gs = g.get_state()
         errs = {k: rel(gs[k], O.interior(st, pr, k)) for k in "uvwe"}
         print("small", kind, istage, acc, ref_acc, tot, cl.last_accept.size, errs, flush=True)
+        # divz comes from the band spectra on this path (k_divz_band)
+        assert np.allclose(g.divz(), st.divz, rtol=1e-8, atol=1e-25)
         assert tot == cl.last_accept.size
         assert abs(acc - ref_acc) <= max(2, 1e-3 * tot)
         tol = 1e-10 if kind == "reject" else 1e-6
