@@ -43,8 +43,9 @@ struct ColShape {
   static constexpr int BYTES = 4 * NY * (int)sizeof(cx<T>);  // per column of the tile
   static constexpr int TK0 = 65536 / BYTES;
   static constexpr int TK = TK0 < MX ? TK0 : MX;             // columns per workgroup
-  // Eight waves per tile: the workgroup's LDS (the padded tile) holds two
-  // workgroups per CU, so the wave count per workgroup sets the occupancy.
+  // Eight waves per tile.  The column kernels pass one batch of TK padded
+  // transforms through LDS at a time (about 32 KB at 256^2 FP32), so four
+  // workgroups share a CU.
   static constexpr int NT = 512;
 };
 
@@ -330,6 +331,12 @@ __global__ void __launch_bounds__(256) k_wall_final(const double* __restrict__ p
 // ---------------------------------------------------------------------------
 // k_col_synth: band spectrum -> 3/2-grid x-spectral rows, value and d/dy
 // grid (MX/TK, nz, 4 fields u,v,w,e)
+//
+// A workgroup holds TK band columns of one level and field in registers and
+// passes the value, then the y-derivative, through one batch of TK
+// transforms in LDS (zero-padded to NYB), so that four workgroups share a
+// CU.  e arrives as rows of physical y: its y-forward transform runs first,
+// in the same batch.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct SynthArgs {
@@ -349,94 +356,87 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
+  constexpr int PLB = padlen(NYB), PLS = padlen(NY);
+  constexpr int LPS = (TK * NY + NT - 1) / NT;        // band values per thread
+  constexpr int NZP = NYB - NY + 1;                    // zero modes NY/2 .. NYB - NY/2
+  static_assert(PLS <= PLB, "the length-NY batch fits the length-NYB batch");
   const int f = blockIdx.z;
   const int z = blockIdx.y + 1;
   if (f >= 2 && z >= a.nz) return;       // w, e live on faces 1..nz-1
   const int k0 = blockIdx.x * TK;
   const int tid = threadIdx.x;
-  __shared__ cx<T> spec[TK * padlen(NY)];
-  __shared__ cx<T> buf[2 * TK * padlen(NYB)];
+  __shared__ cx<T> buf[TK * PLB];
   __shared__ cx<T> twb[NYB];
   __shared__ cx<T> tws[NY];
   load_twiddles(twb, a.twNYB, NYB, tid, NT);
-
+  // element idx = tid + q NT of the tile: column idx % TK, band row idx / TK
+  cx<T> sv[LPS];
   if (f < 3) {
     const cx<T>* src = a.S[f] + (size_t)z * NY * MX + k0;
 #pragma unroll
-    for (int i = tid; i < TK * NY; i += NT) {
-      const int c = i % TK, j = i / TK;
-      spec[c * padlen(NY) + pidx(j)] = src[(size_t)j * MX + c];
+    for (int q = 0; q < LPS; ++q) {
+      const int idx = tid + q * NT;
+      if (idx < TK * NY) sv[q] = src[(size_t)(idx / TK) * MX + idx % TK];
     }
-    __syncthreads();
   } else {
     load_twiddles(tws, a.twNY, NY, tid, NT);
     const cx<T>* src = a.Ex + (size_t)z * NY * MX + k0;
-#pragma unroll
     for (int i = tid; i < TK * NY; i += NT) {
       const int c = i % TK, y = i / TK;
-      spec[c * padlen(NY) + pidx(y)] = src[(size_t)y * MX + c];
+      buf[c * PLS + pidx(y)] = src[(size_t)y * MX + c];
     }
     __syncthreads();
-    fft_waves<T, NY, false, TK, NT / 64>(spec, tws, tid);
+    fft_waves<T, NY, false, TK, NT / 64>(buf, tws, tid);
     __syncthreads();
     const T s = T(1) / T(NY);
 #pragma unroll
-    for (int i = tid; i < TK * NY; i += NT) {
-      const int c = i / NY, j = i % NY;
-      cx<T>& v = spec[c * padlen(NY) + pidx(j)];
-      v = (j == NY / 2) ? cx<T>{0, 0} : cscale(v, s);
+    for (int q = 0; q < LPS; ++q) {
+      const int idx = tid + q * NT;
+      if (idx < TK * NY) {
+        const int c = idx % TK, j = idx / TK;
+        sv[q] = (j == NY / 2) ? cx<T>{0, 0} : cscale(buf[c * PLS + pidx(j)], s);
+      }
     }
-    __syncthreads();
+    __syncthreads();                     // buf is refilled below
   }
-  if (a.lean && f < 3) {                 // the value only
-    for (int i = tid; i < TK * NYB; i += NT) {
-      const int c = i / NYB, n = i - c * NYB;
-      int j = -1;
-      if (n < NY / 2) j = n;
-      else if (n > NYB - NY / 2) j = n - NYB + NY;
-      buf[c * padlen(NYB) + pidx(n)] = j >= 0 ? spec[c * padlen(NY) + pidx(j)] : cx<T>{0, 0};
+  cx<T>* o = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
+  // pass d = 0: the value, d = 1: d/dy (i ky S).  Two calls rather than a
+  // loop: a loop lets the compiler hoist the transform's twiddle reads out of
+  // it and hold them in registers across both passes.
+  auto pass = [&](int d) {
+#pragma unroll
+    for (int q = 0; q < LPS; ++q) {
+      const int idx = tid + q * NT;
+      if (idx < TK * NY) {
+        const int c = idx % TK, j = idx / TK;
+        if (j != NY / 2) {               // zero-pad the band into the 3/2 grid
+          cx<T> v = sv[q];
+          if (d) {
+            const T k = a.kyp[j];
+            v = cx<T>{-k * v.y, k * v.x};
+          }
+          buf[c * PLB + pidx(j < NY / 2 ? j : j - NY + NYB)] = v;
+        }
+      }
+    }
+    for (int i = tid; i < TK * NZP; i += NT) {
+      const int c = i / NZP;
+      buf[c * PLB + pidx(NY / 2 + (i - c * NZP))] = cx<T>{0, 0};
     }
     __syncthreads();
     fft_waves<T, NYB, true, TK, NT / 64>(buf, twb, tid);
     __syncthreads();
-    cx<T>* o0 = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
+    cx<T>* od = o + (size_t)d * a.bstride;
+#pragma unroll
     for (int i = tid; i < TK * NYB; i += NT) {
       const int c = i % TK, n = i / TK;
-      o0[(size_t)n * MX + c] = buf[c * padlen(NYB) + pidx(n)];
+      od[(size_t)n * MX + c] = buf[c * PLB + pidx(n)];
     }
-    return;
-  }
-  // zero-pad the band into the 3/2 grid; second half of the batch holds i ky S
-#pragma unroll
-  for (int i = tid; i < 2 * TK * NYB; i += NT) {
-    const int t = i / NYB, n = i - t * NYB;
-    const int c = t % TK;
-    int j = -1;
-    if (n < NY / 2) j = n;
-    else if (n > NYB - NY / 2) j = n - NYB + NY;
-    cx<T> v = {0, 0};
-    if (j >= 0) {
-      v = spec[c * padlen(NY) + pidx(j)];
-      if (t >= TK) {
-        const T k = a.kyp[j];
-        v = cx<T>{-k * v.y, k * v.x};
-      }
-    }
-    buf[t * padlen(NYB) + pidx(n)] = v;
-  }
-  __syncthreads();
-  fft_waves<T, NYB, true, 2 * TK, NT / 64>(buf, twb, tid);
-  __syncthreads();
-  cx<T>* o0 = a.B + (size_t)(2 * f) * a.bstride + (size_t)z * NYB * MX + k0;
-  cx<T>* o1 = o0 + a.bstride;
-#pragma unroll
-  for (int i = tid; i < 2 * TK * NYB; i += NT) {
-    const int c = i % TK;
-    const int r = i / TK;
-    const int n = r % NYB;
-    const int d = r / NYB;
-    (d ? o1 : o0)[(size_t)n * MX + c] = buf[(d * TK + c) * padlen(NYB) + pidx(n)];
-  }
+  };
+  pass(0);
+  if (a.lean && f < 3) return;
+  __syncthreads();                       // buf is refilled by the second pass
+  pass(1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1901,7 +1901,14 @@ __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // k_col_anal: y-forward of the flux rows, band truncation, R = G + i ky Fy.
-// grid (MX/TK, nz, 4 components u,v,w,e)
+// One tile per workgroup: TK kx columns of one level and component; tiles
+// kx fastest, then level, then component (u, v, w, e).
+//
+// G and F^y pass one after the other through one batch of TK transforms in
+// LDS, so that four workgroups share a CU; both tiles are loaded into
+// registers up front, so F^y's reads are in flight during G's transform.
+// For e the band result takes a y-inverse (to x-spectral rows of physical y)
+// in the same batch.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct AnalArgs {
@@ -1915,81 +1922,113 @@ struct AnalArgs {
   int nz;
 };
 
-// One tile per workgroup: TK kx columns of one level and component.  (A
-// persistent loop over tiles with the next tile's columns held in registers
-// took 230-400 VGPRs, one wave per SIMD, and measured 37 % slower.)
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
-  constexpr int NKT = MX / TK;
-  constexpr int LPT = 2 * TK * NYB / NT;       // tile elements per thread
-  static_assert((2 * TK * NYB) % NT == 0, "tile must divide over the workgroup");
   constexpr int PLB = padlen(NYB), PLS = padlen(NY);
+  constexpr int NKT = MX / TK;
+  constexpr int LPT = (TK * NYB + NT - 1) / NT;      // tile elements per thread
+  constexpr int OPT = (TK * NY + NT - 1) / NT;       // band outputs per thread
+  constexpr bool FULL = (TK * NYB) % NT == 0;
+  static_assert(PLS <= PLB, "the length-NY batch fits the length-NYB batch");
   const int tid = threadIdx.x;
-  __shared__ cx<T> spec[TK * PLS];
-  __shared__ cx<T> buf[2 * TK * PLB];
+  __shared__ cx<T> buf[TK * PLB];
   __shared__ cx<T> twb[NYB];
   __shared__ cx<T> tws[NY];
-  const size_t splane = (size_t)NY * MX;
   const int nz = a.nz;
-  const int ntiles = NKT * nz * 4;
-  load_twiddles(twb, a.twNYB, NYB, tid, NT);
-  load_twiddles(tws, a.twNY, NY, tid, NT);
+  const size_t splane = (size_t)NY * MX;
   const int t = blockIdx.x;
-  if (t < ntiles) {
-    const int kt = t % NKT, rest = t / NKT;
-    const int z = rest % nz + 1, cmp = rest / nz;
-    const int k0 = kt * TK;
-    if (cmp >= 2 && z == nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
-      cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
-      for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
-      return;
-    }
-    {
-      const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + k0;
+  if (t >= NKT * nz * 4) return;
+  const int kt = t % NKT, rest = t / NKT;
+  const int z = rest % nz + 1, cmp = rest / nz;
+  const int k0 = kt * TK;
+  if (cmp >= 2 && z == nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
+    cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
+    for (int i = tid; i < TK * NY; i += NT) o[(size_t)(i / TK) * MX + (i % TK)] = cx<T>{0, 0};
+    return;
+  }
+  load_twiddles(twb, a.twNYB, NYB, tid, NT);
+  if (cmp == 3) load_twiddles(tws, a.twNY, NY, tid, NT);
+  const cx<T>* g = a.A + (size_t)(2 * cmp) * a.astride + (size_t)z * NYB * MX + k0;
+  const cx<T>* fy = g + a.astride;
+  cx<T> lg[LPT], lf[LPT];
 #pragma unroll
-      for (int i = 0; i < LPT; ++i) {
-        const int idx = tid + i * NT;
-        const int c = idx % TK, r = idx / TK;
-        const int n = r % NYB, d = r / NYB;
-        buf[(d * TK + c) * PLB + pidx(n)] = g[(size_t)d * a.astride + (size_t)n * MX + c];
-      }
+  for (int i = 0; i < LPT; ++i) {
+    const int idx = tid + i * NT;
+    if (FULL || idx < TK * NYB) {
+      const int c = idx % TK, n = idx / TK;
+      lg[i] = g[(size_t)n * MX + c];
+      lf[i] = fy[(size_t)n * MX + c];
     }
-    __syncthreads();
-    fft_waves<T, NYB, false, 2 * TK, NT / 64>(buf, twb, tid);
-    __syncthreads();
-    const T s = T(1) / T(NYB);
+  }
+  auto band_row = [](int j) { return (j < NY / 2) ? j : j - NY + NYB; };
 #pragma unroll
-    for (int i = tid; i < TK * NY; i += NT) {
+  for (int i = 0; i < LPT; ++i) {
+    const int idx = tid + i * NT;
+    if (FULL || idx < TK * NYB) buf[(idx % TK) * PLB + pidx(idx / TK)] = lg[i];
+  }
+  __syncthreads();
+  fft_waves<T, NYB, false, TK, NT / 64>(buf, twb, tid);
+  __syncthreads();
+  cx<T> v[OPT];                              // G, then the band result
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int i = tid + q * NT;
+    v[q] = cx<T>{0, 0};
+    if (i < TK * NY) {
       const int c = i % TK, j = i / TK;
-      cx<T> v = {0, 0};
-      if (j != NY / 2) {
-        const int n = (j < NY / 2) ? j : j - NY + NYB;
-        const cx<T> gv = buf[c * PLB + pidx(n)];
-        const cx<T> fv = buf[(TK + c) * PLB + pidx(n)];
-        const T k = a.kyp[j];
-        v = cx<T>{(gv.x - k * fv.y) * s, (gv.y + k * fv.x) * s};
-        if (cmp == 2 && j == 0 && k0 + c == 0) v = cx<T>{0, 0};   // rw mean removal (comp1.f90:172)
-      }
-      if (cmp < 3) {
-        a.R[cmp][(size_t)z * splane + (size_t)j * MX + k0 + c] = v;
-      } else {
-        spec[c * PLS + pidx(j)] = v;
-      }
+      if (j != NY / 2) v[q] = buf[c * PLB + pidx(band_row(j))];
     }
-    if (cmp == 3) {
-      __syncthreads();
-      fft_waves<T, NY, true, TK, NT / 64>(spec, tws, tid);
-      __syncthreads();
-      cx<T>* o = a.Tre + (size_t)z * splane + k0;
+  }
+  __syncthreads();
 #pragma unroll
-      for (int i = tid; i < TK * NY; i += NT) {
-        const int c = i % TK, y = i / TK;
-        o[(size_t)y * MX + c] = spec[c * PLS + pidx(y)];
+  for (int i = 0; i < LPT; ++i) {
+    const int idx = tid + i * NT;
+    if (FULL || idx < TK * NYB) buf[(idx % TK) * PLB + pidx(idx / TK)] = lf[i];
+  }
+  __syncthreads();
+  fft_waves<T, NYB, false, TK, NT / 64>(buf, twb, tid);
+  __syncthreads();
+  const T s = T(1) / T(NYB);
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int i = tid + q * NT;
+    if (i < TK * NY) {
+      const int c = i % TK, j = i / TK;
+      if (j != NY / 2) {
+        const cx<T> fv = buf[c * PLB + pidx(band_row(j))];
+        const T k = a.kyp[j];
+        v[q] = cx<T>{(v[q].x - k * fv.y) * s, (v[q].y + k * fv.x) * s};
+        if (cmp == 2 && j == 0 && k0 + c == 0) v[q] = cx<T>{0, 0};   // rw mean removal (comp1.f90:172)
       }
     }
+  }
+  if (cmp < 3) {
+    cx<T>* R = a.R[cmp] + (size_t)z * splane + k0;
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int i = tid + q * NT;
+      if (i < TK * NY) R[(size_t)(i / TK) * MX + i % TK] = v[q];
+    }
+    return;
+  }
+  // re: the y-inverse of the band result to rows of physical y
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int i = tid + q * NT;
+    if (i < TK * NY) buf[(i % TK) * PLS + pidx(i / TK)] = v[q];
+  }
+  __syncthreads();
+  fft_waves<T, NY, true, TK, NT / 64>(buf, tws, tid);
+  __syncthreads();
+  cx<T>* o = a.Tre + (size_t)z * splane + k0;
+#pragma unroll
+  for (int i = tid; i < TK * NY; i += NT) {
+    const int c = i % TK, y = i / TK;
+    o[(size_t)y * MX + c] = buf[c * PLS + pidx(y)];
   }
 }
 

@@ -91,6 +91,7 @@ struct cf_ctx {
   hipStream_t stream2 = nullptr;    // the small-grid closure's columns-in and rows run here,
   hipEvent_t ev_fork = nullptr;     // concurrently with k_col_synth .. k_col_anal
   hipEvent_t ev_join = nullptr;
+  bool slot_used[CF_NKERN] = {};    // per-kernel timing: which slots launched work
   double* wpart = nullptr;      // isurface 2..5: wall-plane sums [2][NYB][8]
   int* dirty = nullptr;         // faces whose expanded e was clamped this stage; [0]: any
   void* eclDX = nullptr;        // clamp residual derivatives [nz+1][NYB][NXB]
@@ -268,8 +269,12 @@ struct EngineT : Engine {
     const size_t band_plane = (size_t)NY * MX;
     const size_t bstride = (size_t)ctx->slots * NYB * MX;
     int ek = 0;
-    auto mark = [&]() {
-      if (ev) (void)hipEventRecord(ev[ek++], s);
+    // records the event that closes timing slot ek-1 (used: whether that slot
+    // launched work on this path) and opens slot ek
+    auto mark = [&](bool used) {
+      if (!ev) return;
+      if (ek > 0) ctx->slot_used[ek - 1] = used;
+      (void)hipEventRecord(ev[ek++], s);
     };
     // The small-grid closure's divz, columns-in and row kernel depend only on
     // the stage-start state and write only their own buffers: they run on the
@@ -279,10 +284,10 @@ struct EngineT : Engine {
     if constexpr (NX % 64 == 0) small = ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL;
     const bool overlap = small && !ev;
     HIPCHK(hipMemsetAsync(ctx->dirty, 0, ((size_t)nz + 3) * sizeof(int), s));
-    mark();
+    mark(false);
     hipLaunchKernelGGL(cf::k_wall<T>, dim3(1), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
                        cptr(ctx->Ruo), cptr(ctx->Rvo), band_plane, c, ctx->ws);
-    mark();
+    mark(true);
     {
       cf::SynthArgs<T> a{};
       a.S[0] = cptr(ctx->U);
@@ -323,7 +328,7 @@ struct EngineT : Engine {
       hipLaunchKernelGGL((cf::k_shear<T, NX>), dim3(NYB, nchs), dim3(RS::NT), 0, s, cptr(ctx->Bs), bstride,
                          rptr(ctx->kxp), cptr(ctx->twNXB), cs, rptr(ctx->X));
     }
-    mark();
+    mark(true);
     {
       cf::RowArgs<T> a{};
       a.B = cptr(ctx->Bs);
@@ -376,7 +381,7 @@ struct EngineT : Engine {
       hipLaunchKernelGGL((cf::k_eclamp_cols<T, NX>), dim3(NXB / 16), dim3(256), 0, s, ea);
       hipLaunchKernelGGL((cf::k_eclamp_flux<T, NX>), dim3(NYB), dim3(RS::NT), 0, s, ea);
     }
-    mark();
+    mark(true);
     {
       cf::AnalArgs<T> a{};
       a.A = cptr(ctx->Ba);
@@ -392,7 +397,7 @@ struct EngineT : Engine {
       const int ntiles = (MX / CS::TK) * nz * 4;      // tiles: kx fastest, then level, then component
       hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(ntiles), dim3(CS::NT), 0, s, a);
     }
-    mark();
+    mark(true);
     if constexpr (NX % 64 == 0) {
       if (small && !overlap) {
         int rc = divz_band(ctx, c.dzm, s);
@@ -400,7 +405,7 @@ struct EngineT : Engine {
         if (rc != CF_OK) return rc;
       }
     }
-    mark();
+    mark(small);
     if constexpr (NX % 64 == 0) {
       if (small) {
         if (overlap) {
@@ -411,14 +416,14 @@ struct EngineT : Engine {
         }
       }
     }
-    mark();
+    mark(small);
     if constexpr (NX % 64 == 0) {
       if (small) {
         int rc = sgs_small_out(ctx, s);
         if (rc != CF_OK) return rc;
       }
     }
-    mark();
+    mark(small);
     {
       cf::PressArgs<T> a{};
       a.U[0] = cptr(ctx->U);
@@ -447,7 +452,7 @@ struct EngineT : Engine {
       const int modes = MX * NY;
       hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 63) / 64), dim3(64), 0, s, a);
     }
-    mark();
+    mark(true);
     {
       cf::ERowArgs<T> a{};
       a.e = rptr(ctx->e);
@@ -464,14 +469,14 @@ struct EngineT : Engine {
       a.emin = c.emin;
       hipLaunchKernelGGL((cf::k_erows<T, NX>), dim3(NY / 2, nz), dim3(MX), 0, s, a);
     }
-    mark();
+    mark(true);
     {
       const int q0 = small ? 1 : 0, nq = sgs ? 7 : 1;        // divz from k_divz_band when small
       if (nq > q0)
         hipLaunchKernelGGL(cf::k_reduce_part, dim3(ctx->slots), dim3(256), 0, s, ctx->part,
                            (size_t)ctx->slots * NYB, NYB, q0, nq, c.xymb, ctx->red, ctx->slots);
     }
-    mark();
+    mark((small ? 1 : 0) < (sgs ? 7 : 1));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       ctx->err = std::string("stage launch: ") + hipGetErrorString(e);
@@ -1008,35 +1013,43 @@ int cf_time_steps(cf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms) 
   HIPCHK(hipEventCreate(&b));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipEventRecord(a, ctx->stream));
-  for (int it = 0; it < nsteps; ++it)
-    for (int s = 1; s <= 3; ++s) {
-      int rc = ctx->eng->stage(ctx, s, 0, nullptr);
-      if (rc != CF_OK) return rc;
-    }
-  HIPCHK(hipEventRecord(b, ctx->stream));
-  HIPCHK(hipEventSynchronize(b));
+  int rc = CF_OK;
+  for (int it = 0; it < nsteps && rc == CF_OK; ++it)
+    for (int s = 1; s <= 3 && rc == CF_OK; ++s) rc = ctx->eng->stage(ctx, s, 0, nullptr);
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  if (rc == CF_OK) {
+    HIPCHK(hipEventRecord(b, ctx->stream));
+    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+  }
   *total_ms = ms;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  if (rc != CF_OK) return rc;
   if (kernel_ms) {
     constexpr int NE = CF_NKERN + 1;
     for (int k = 0; k < CF_NKERN; ++k) kernel_ms[k] = 0;
-    std::vector<hipEvent_t> ev(NE);
+    std::vector<hipEvent_t> ev(NE, nullptr);
+    struct Destroy {
+      std::vector<hipEvent_t>& v;
+      ~Destroy() {
+        for (auto& x : v)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } destroy{ev};
     for (auto& x : ev) HIPCHK(hipEventCreate(&x));
     for (int it = 0; it < nsteps; ++it)
       for (int s = 1; s <= 3; ++s) {
-        int rc = ctx->eng->stage(ctx, s, 0, ev.data());
+        rc = ctx->eng->stage(ctx, s, 0, ev.data());
         if (rc != CF_OK) return rc;
         HIPCHK(hipEventSynchronize(ev[NE - 1]));
         for (int k = 0; k < NE - 1; ++k) {
+          if (!ctx->slot_used[k]) continue;      // no kernel of this slot on this path
           float t = 0;
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
           kernel_ms[k] += t / (3.0 * nsteps);
         }
       }
-    for (auto& x : ev) (void)hipEventDestroy(x);
   }
   return check_flags(ctx);
 }
