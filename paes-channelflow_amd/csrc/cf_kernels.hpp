@@ -2062,10 +2062,12 @@ __device__ __forceinline__ void st2(cx<T>* p, cx<double> v) {
 
 // The z sweeps are latency bound (one dependent step per level), so each
 // thread loads the inputs of ZU levels before running their ZU recurrence
-// steps: ZU levels of loads are in flight per memory round trip.
+// steps: ZU levels of loads are in flight per memory round trip.  FP32
+// inputs take ZU = 8 (218 VGPRs; 512 wavefronts at 256^2 leave the register
+// file free); FP64 inputs stay at 4 (191 VGPRs).
 template <typename T>
 __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
-  constexpr int ZU = 4;
+  constexpr int ZU = sizeof(T) == 4 ? 8 : 4;
   const int modes = a.mx * a.ny;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= modes) return;
