@@ -16,7 +16,8 @@ through the C-ABI.
   relative L2, and the same cells accept within 0.1 %.  With a network that
   every cell rejects, fields agree within 1e-10 of the TKE stage.
 * The same with the network on the nnx x nny grid, against
-  sgs_mlp.LearnedClosureSmall, plus the stage-1 SGS mean profiles.
+  sgs_mlp.LearnedClosureSmall, plus the stage-1 SGS mean profiles; and that
+  mode's two-stream stage against its one-stream stage (bit-identical).
 """
 import numpy as np
 import pytest
@@ -235,6 +236,48 @@ def test_small_grid_closure_matches_oracle(kind, nz, stages):
                 d = float(np.max(np.abs(mg[i] - mo[i]))) / max(float(np.max(np.abs(mo[i]))), 1e-6 * scale)
                 print("  mean", O.MEAN_NAMES[i], f"{d:.2e}", flush=True)
                 assert d < (1e-9 if kind == "reject" else 1e-2), (O.MEAN_NAMES[i], d)
+    g.close()
+    net.close()
+
+
+def test_two_stream_stage_equals_one_stream_stage():
+    """In the nnx x nny mode the closure's divz, columns-in and row kernels run
+    on a second stream beside the 3/2-grid sweep; the per-kernel timing pass of
+    cf_time_steps runs the same stages on one stream.  Four steps either way
+    give bit-identical fields (no kernel feeding the fields uses atomics).  The
+    timing pass reports the closure kernels only on a path that launches them."""
+    import channelflow as C
+    pr = O.Params(128, 128, 16, dt=0.002)
+    u, v, w, e = O.initial_fields(pr)
+    flat, norm = _closure_net("mixed")
+    net = make_net(flat, norm)
+    states = []
+    for per_kernel in (False, True):
+        g = _les(pr, C.CF_PREC_FP32)
+        g.set_state(u, v, w, e)
+        g.set_closure(net)
+        g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL)
+        if per_kernel:
+            _, pk = g.time_steps(2, per_kernel=True)        # 2 two-stream steps, then 2 one-stream steps
+            print("small-grid per-kernel ms", pk, flush=True)
+            for k in ("k_col_synth", "k_rows", "k_col_anal", "k_sgs_cols_in", "k_sgs_rows", "k_sgs_cols_out",
+                      "k_press", "k_erows"):
+                assert pk[k] > 0, k
+        else:
+            g.time_steps(2)
+            g.time_steps(2)
+        states.append(g.get_state())
+        g.close()
+    for k in "uvwe":
+        assert np.array_equal(states[0][k], states[1][k]), k
+    g = _les(pr, C.CF_PREC_FP64)
+    g.set_state(u, v, w, e)
+    _, pk = g.time_steps(1, per_kernel=True)
+    print("plain per-kernel ms", pk, flush=True)
+    for k in ("k_sgs_cols_in", "k_sgs_rows", "k_sgs_cols_out"):
+        assert pk[k] == 0, k
+    for k in ("k_col_synth", "k_rows", "k_col_anal", "k_press", "k_erows", "k_reduce_part"):
+        assert pk[k] > 0, k
     g.close()
     net.close()
 
