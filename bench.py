@@ -41,15 +41,21 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_GPSTEP = {"fp32": 436.0, "fp64": 776.0}
 
 
-def kernel_bytes(name, n, s, istage=2):
+def kernel_bytes(name, n, s, istage=2, lean=False):
     """Bytes one launch of kernel ``name`` must read and write, for N = n grid
     points and s-byte reals (complex = 2s; band spectra and x-spectral rows of
-    the nnx x nny plane hold N*s bytes per field, 3/2-grid rows 1.5*N*s)."""
+    the nnx x nny plane hold N*s bytes per field, 3/2-grid rows 1.5*N*s).
+    ``lean``: the 3/2-grid sweep of the nnx x nny closure mode (no velocity
+    gradients)."""
     return {
         # U, V, W band spectra + x-spectral e in; u,uy,v,vy,w,wy,e,ey rows out
-        "k_col_synth": 4 * n * s + 12 * n * s,
+        # (lean: u, v, w, e, ey)
+        "k_col_synth": 4 * n * s + (7.5 if lean else 12) * n * s,
+        # U, V, W of the level and W of the level below in (band spectra)
+        "k_divz_band": 4 * n * s,
         # 8 synthesis rows in, 8 flux rows (Gu,Fyu,Gv,Fyv,Gw,Fyw,Ge,Fye) out
-        "k_rows": 12 * n * s + 12 * n * s,
+        # (lean: u, v, w, e, ey in)
+        "k_rows": (7.5 if lean else 12) * n * s + 12 * n * s,
         # 8 flux rows in; ru, rv, rw band spectra + x-spectral re out
         "k_col_anal": 12 * n * s + 4 * n * s,
         # U,V,W, R_old, R_new in; U,V,W, R out; FP64 pressure spectrum out
@@ -237,10 +243,11 @@ def main():
     dom = max((k for k in kern if not k.startswith("(")), key=lambda k: kern[k])
     step_ms = r32["ms"] / args.steps
     step_achieved = BYTES_PER_GPSTEP["fp32"] * value / 1e9
-    dom_bytes = kernel_bytes(dom, pts, 4)
+    lean = args.closure_grid == "small"
+    dom_bytes = kernel_bytes(dom, pts, 4, lean=lean)
     achieved = dom_bytes / (kern[dom] * 1e-3) / 1e9
-    per_kernel_gbs = {k: kernel_bytes(k, pts, 4) / (t * 1e-3) / 1e9
-                      for k, t in kern.items() if kernel_bytes(k, pts, 4) and t > 0}
+    per_kernel_gbs = {k: kernel_bytes(k, pts, 4, lean=lean) / (t * 1e-3) / 1e9
+                      for k, t in kern.items() if kernel_bytes(k, pts, 4, lean=lean) and t > 0}
     # on the 3/2 grid the learned path runs the row kernel with the network
     # inside (k_rows_mlp): same row inputs and outputs, same algorithmic bytes
     dom_label = "k_rows_mlp" if dom == "k_rows" and args.closure_grid == "dealiased" else dom

@@ -147,9 +147,11 @@ int cf_calc_spectra(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* h
  * milliseconds and, when kernel_ms is not NULL, the average duration of each
  * kernel family of one step measured with events bracketing every launch
  * (kernel_ms[CF_NKERN]; the per-kernel pass runs after the total timing). */
-/* k_wall k_col_synth k_rows k_col_anal | k_sgs_cols_in k_sgs_rows k_sgs_cols_out
- * (the small-grid closure: zero time otherwise) | k_press k_erows k_reduce_part */
-#define CF_NKERN 10
+/* k_wall k_col_synth k_divz_band k_rows k_col_anal | k_sgs_cols_in k_sgs_rows
+ * k_sgs_cols_out (the small-grid closure: zero time otherwise) | k_press k_erows
+ * k_reduce_part (the SGS means of an mnout stage: zero time otherwise).  A family
+ * no kernel of which runs on the path reports 0. */
+#define CF_NKERN 11
 int cf_time_steps(cf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms);
 
 /* Name of kernel family k of cf_time_steps (0 <= k < CF_NKERN). */

@@ -32,7 +32,7 @@ CF_E_ECLAMP = -5
 CF_E_NODEV = -6
 CF_PREC_FP64 = 0
 CF_PREC_FP32 = 1
-CF_NKERN = 10
+CF_NKERN = 11
 
 EXPORTS = (
     "cf_create", "cf_destroy", "cf_set_state", "cf_seeded_init", "cf_get_state", "cf_get_tendency",

@@ -1050,28 +1050,24 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
       fb[3] = Fyw;
       fa[4] = Fxe; fb[4] = He;
       fa[5] = Fye; fb[5] = T(0);
-      const double dv = (double)(ux + vy + wz0);                  // comp1.f90:145
-      sums[0] = LEAN ? 0.0 : dv * dv;
+      sums[0] = 0.0;                   // divz (comp1.f90:145) comes from k_divz_band
       sums[1] = (double)tuu; sums[2] = (double)tvv; sums[3] = (double)tuv;
       sums[4] = (double)tww0; sums[5] = (double)tuw0; sums[6] = (double)tvw0;
     }
-    // partial sums of divz and (when means are due) the SGS stress means
+    // partial sums of the SGS stress means, when means are due (slot 0, divz,
+    // stays unused: k_divz_band computes it from the band spectra)
     if constexpr (F::WAVE) {
-      const int nq = mn ? 7 : 1;
-      for (int q = 0; q < nq; ++q) {
+      const int nq = mn ? 7 : 0;
+      for (int q = 1; q < nq; ++q) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sums[q] += __shfl_xor(sums[q], off, 64);
       }
       if ((tid & 63) == 0)
-        for (int q = 0; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
+        for (int q = 1; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
     } else {
       if (mn) {
         block_sum<NT, 7>(sums, red, tid);
         if (tid < 7) a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sums[0];
-      } else {
-        double d1[1] = {sums[0]};
-        block_sum<NT, 1>(d1, red, tid);
-        if (tid == 0) a.part[(size_t)iz * NYB + y] = d1[0];
       }
     }
     {
@@ -1096,8 +1092,8 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     }
     if constexpr (F::WAVE) {
       // wave partials were published by the barriers inside anal_pairs
-      const int nq = mn ? 7 : 1;
-      if (tid < nq) {
+      const int nq = mn ? 7 : 0;
+      if (tid >= 1 && tid < nq) {
         double sq = 0;
         for (int w = 0; w < F::NW; ++w) sq += red[tid * F::NW + w];
         a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sq;
@@ -1328,21 +1324,20 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
         fb[3] = Fyw;
         fa[4] = Fxe; fb[4] = He;
         fa[5] = Fye; fb[5] = T(0);
-        const double dv = (double)(ux0 + vy0 + wz0);              // comp1.f90:145
-        sums[0] = dv * dv;
+        sums[0] = 0.0;                 // divz (comp1.f90:145) comes from k_divz_band
         sums[1] = (double)tuu; sums[2] = (double)tvv; sums[3] = (double)tuv;
         sums[4] = (double)tww0; sums[5] = (double)tuw0; sums[6] = (double)tvw0;
       }
       const unsigned long long bal = __ballot(ok0);
       if (lane == 0) nacc += (unsigned long long)__popcll(bal);
       {
-        const int nq = mn ? 7 : 1;
-        for (int q = 0; q < nq; ++q) {
+        const int nq = mn ? 7 : 0;                 // the SGS means; divz from k_divz_band
+        for (int q = 1; q < nq; ++q) {
 #pragma unroll
           for (int off = 32; off > 0; off >>= 1) sums[q] += __shfl_xor(sums[q], off, 64);
         }
         if (lane == 0)
-          for (int q = 0; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
+          for (int q = 1; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
       }
       {
         cx<T>* A = a.A;
@@ -1365,8 +1360,8 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
         anal_pairs<T, NX, 6>(buf, fa, fb, tw, tid, emit);
       }
       {
-        const int nq = mn ? 7 : 1;
-        if (tid < nq) {
+        const int nq = mn ? 7 : 0;
+        if (tid >= 1 && tid < nq) {
           double sq = 0;
           for (int w = 0; w < F::NW; ++w) sq += red[tid * F::NW + w];
           a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sq;

@@ -42,9 +42,9 @@ thread_local std::string g_create_error;
 
 constexpr double PI = 3.14159265358979323846264338327950288;
 
-const char* kKernelNames[CF_NKERN] = {"k_wall",     "k_col_synth",    "k_rows",  "k_col_anal",
-                                      "k_sgs_cols_in", "k_sgs_rows", "k_sgs_cols_out",
-                                      "k_press",    "k_erows",        "k_reduce_part"};
+const char* kKernelNames[CF_NKERN] = {"k_wall",        "k_col_synth", "k_divz_band",    "k_rows",
+                                      "k_col_anal",    "k_sgs_cols_in", "k_sgs_rows", "k_sgs_cols_out",
+                                      "k_press",       "k_erows",     "k_reduce_part"};
 
 }  // namespace
 
@@ -329,6 +329,11 @@ struct EngineT : Engine {
                          rptr(ctx->kxp), cptr(ctx->twNXB), cs, rptr(ctx->X));
     }
     mark(true);
+    if (!overlap) {               // divz (comp1.f90:145) by Parseval from the band spectra
+      int rc = divz_band(ctx, c.dzm, s);
+      if (rc != CF_OK) return rc;
+    }
+    mark(true);
     {
       cf::RowArgs<T> a{};
       a.B = cptr(ctx->Bs);
@@ -400,8 +405,7 @@ struct EngineT : Engine {
     mark(true);
     if constexpr (NX % 64 == 0) {
       if (small && !overlap) {
-        int rc = divz_band(ctx, c.dzm, s);
-        if (rc == CF_OK) rc = sgs_cols_in(ctx, s);
+        int rc = sgs_cols_in(ctx, s);
         if (rc != CF_OK) return rc;
       }
     }
@@ -471,12 +475,12 @@ struct EngineT : Engine {
     }
     mark(true);
     {
-      const int q0 = small ? 1 : 0, nq = sgs ? 7 : 1;        // divz from k_divz_band when small
+      const int q0 = 1, nq = sgs ? 7 : 1;        // the SGS means (slot 0, divz: k_divz_band)
       if (nq > q0)
         hipLaunchKernelGGL(cf::k_reduce_part, dim3(ctx->slots), dim3(256), 0, s, ctx->part,
                            (size_t)ctx->slots * NYB, NYB, q0, nq, c.xymb, ctx->red, ctx->slots);
     }
-    mark((small ? 1 : 0) < (sgs ? 7 : 1));
+    mark(sgs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       ctx->err = std::string("stage launch: ") + hipGetErrorString(e);
@@ -515,14 +519,10 @@ struct EngineT : Engine {
     return CF_OK;
   }
 
-  // The learned closure on the nnx x nny grid: y-inverse of u, v, w and their
-  // y-derivatives to x-spectral small rows, the row kernel, then the
-  // y-forward of its divergence rows added to the comp1 tendencies.
-  // The learned closure on the nnx x nny grid: sgs_cols_in (the y-inverse of
-  // U, V, W and their y-derivatives to x-spectral small rows), sgs_rows (the
-  // row kernel: network, TKE fallback, divergence rows) and sgs_small_out.
-  // divz from the stage-start band spectra (the lean row kernel does not
-  // synthesise the velocity gradients); into the q = 0 slots of red.
+  // divz (comp1.f90:145) on every path: the plane mean of (ux + vy + wz)^2 on
+  // the 3/2 grid, by Parseval from the stage-start band spectra (the square of
+  // a band-limited field is resolved there, so the two agree to rounding);
+  // into the q = 0 slots of red.
   int divz_band(cf_ctx* ctx, double dzm, hipStream_t s) {
     HIPCHK(hipMemsetAsync(ctx->red, 0, (size_t)ctx->slots * sizeof(double), s));   // the q = 0 slots
     const int parts = std::max(1, std::min(16, (MX * NY) / 2048));
@@ -532,6 +532,10 @@ struct EngineT : Engine {
     return CF_OK;
   }
 
+  // The learned closure on the nnx x nny grid: sgs_cols_in (the y-inverse of
+  // U, V, W and their y-derivatives to x-spectral small rows), sgs_rows (the
+  // row kernel: network, TKE fallback, divergence rows) and sgs_small_out
+  // (the y-forward of the divergence rows, added to the comp1 tendencies).
   int sgs_cols_in(cf_ctx* ctx, hipStream_t s) {
     constexpr int TK = CS::TK;
     cf::SgsColsInArgs<T> a{};
