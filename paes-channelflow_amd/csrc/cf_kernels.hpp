@@ -584,6 +584,76 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
   }
 }
 
+// Half spectra that a wave fills into its synthesis transforms q = w*TPW + qq
+// (modes k = lane + 64 kk < MX), loaded a level ahead so that their memory
+// latency overlaps the work of the current level.
+template <typename T, int MX, int TPW>
+struct RowPre {
+  static constexpr int KPL = (MX + 63) / 64;
+  cx<T> a[TPW][KPL], b[TPW][KPL];
+};
+
+template <typename T, int MX, int TPW, int NP>
+__device__ __forceinline__ void row_prefetch(RowPre<T, MX, TPW>& P, const cx<T>* const* sa,
+                                             const cx<T>* const* sb, int tid) {
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int qq = 0; qq < TPW; ++qq) {
+    const int q = w * TPW + qq;
+    const cx<T>* pa = pick<NP>(sa, q, (const cx<T>*)nullptr);
+    const cx<T>* pb = pick<NP>(sb, q, (const cx<T>*)nullptr);
+#pragma unroll
+    for (int kk = 0; kk < RowPre<T, MX, TPW>::KPL; ++kk) {
+      const int k = lane + 64 * kk;
+      P.a[qq][kk] = (pa && k < MX) ? pa[k] : cx<T>{0, 0};
+      P.b[qq][kk] = (pb && k < MX) ? pb[k] : cx<T>{0, 0};
+    }
+  }
+}
+
+// synth_pairs (wave-owned transforms of length L) from prefetched half
+// spectra: the same spectrum, transform and result layout.
+template <typename T, int L, int MX, int TPW, int NP>
+__device__ __forceinline__ void synth_pre(cx<T>* buf, const RowPre<T, MX, TPW>& P, const bool* da,
+                                          const bool* db, T* ra, T* rb, const T* __restrict__ kxp,
+                                          const cx<T>* tw, int tid) {
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int qq = 0; qq < TPW; ++qq) {
+    const int q = w * TPW + qq;
+    if (q < NP) {
+      cx<T>* z = buf + q * padlen(L);
+      const bool dqa = pick<NP>(da, q, false), dqb = pick<NP>(db, q, false);
+#pragma unroll
+      for (int kk = 0; kk < RowPre<T, MX, TPW>::KPL; ++kk) {
+        const int k = lane + 64 * kk;
+        if (k < MX) {
+          cx<T> av = P.a[qq][kk], bv = P.b[qq][kk];
+          const T kx = kxp[k];
+          if (dqa) av = cx<T>{-kx * av.y, kx * av.x};
+          if (dqb) bv = cx<T>{-kx * bv.y, kx * bv.x};
+          if (k == 0) {
+            z[0] = cx<T>{av.x, bv.x};      // c2r ignores the imaginary part of the DC term
+          } else {
+            z[pidx(k)] = cx<T>{av.x - bv.y, av.y + bv.x};
+            z[pidx(L - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
+          }
+        }
+      }
+      for (int i = MX + lane; i <= L - MX; i += 64) z[pidx(i)] = cx<T>{0, 0};
+    }
+  }
+  wave_sync();
+  if (w * TPW < NP) fft_run<T, L, true, TPW, 64, 0, true>(buf + w * TPW * padlen(L), tw, lane);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const cx<T> v = buf[q * padlen(L) + pidx(tid)];
+    ra[q] = v.x;
+    rb[q] = v.y;
+  }
+}
+
 template <typename T, int NX>
 __global__ void __launch_bounds__(3 * NX / 2) k_wall_plane(const cx<T>* __restrict__ Bs, size_t bstride,
                                                            const T* __restrict__ kxp,
@@ -860,8 +930,17 @@ __global__ void __launch_bounds__(3 * NX / 2) k_shear(const cx<T>* __restrict__ 
 // momentum stresses (Consts::sgs_momentum == 0) and k_divz_band the
 // divergence, so the velocity gradients are not synthesised: three pairs per
 // level instead of six (u, v; w, e; ex, ey).
+// PRE: the synthesis half spectra are loaded a level ahead.  Measured and
+// tuned for FP32 at nnx = 256, where it fits four waves per SIMD (128 VGPRs,
+// no spill; left alone the compiler takes 133-136 and three waves).  Other
+// shapes keep the direct loads: there the extra registers spill or cost a
+// wave per SIMD.
+template <typename T, int NX>
+constexpr bool rows_prefetch() { return sizeof(T) == 4 && NX == 256; }
+
 template <typename T, int NX, bool LEAN = false>
-__global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
+__global__ void __launch_bounds__(3 * NX / 2) __attribute__((amdgpu_waves_per_eu(rows_prefetch<T, NX>() ? 4 : 1)))
+k_rows(RowArgs<T> a) {
   using F = RowFFT<T, NX>;
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
   constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
@@ -949,18 +1028,38 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
     sa[5] = (iz <= nz - 1) ? row(FE, ie) : nullptr;
     sb[5] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
   };
+  // wave-owned row transforms take their half spectra a level ahead
+  constexpr bool PRE = F::WAVE && rows_prefetch<T, NX>();
+  constexpr int NPS = LEAN ? 3 : 6;                 // synthesis pairs per level
+  RowPre<T, MX, F::TPW> P;
+  auto prefetch = [&](int iz) {
+    const cx<T>* sa[6];
+    const cx<T>* sb[6];
+    level_srcs(iz, sa, sb);
+    if constexpr (LEAN) {                           // (u, v) (w, e) (ex, ey)
+      sa[2] = sa[5];
+      sb[2] = sb[5];
+    }
+    row_prefetch<T, MX, F::TPW, NPS>(P, sa, sb, tid);
+  };
+  if constexpr (PRE) prefetch(z0);
   for (int iz = z0; iz < z1; ++iz) {
     T ux, uy, vx, vy, wx0, wy0, ex, ey;
     if constexpr (LEAN) {
       T ra[3], rb[3];
-      const cx<T>* sa[6];
-      const cx<T>* sb[6];
       const bool da[3] = {false, false, true};
       const bool db[3] = {false, false, false};
-      level_srcs(iz, sa, sb);
-      sa[2] = sa[5];
-      sb[2] = sb[5];
-      synth_pairs<T, NX, 3>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      if constexpr (PRE) {
+        synth_pre<T, NXB, MX, F::TPW, 3>(buf, P, da, db, ra, rb, a.kxp, tw, tid);
+        if (iz + 1 < z1) prefetch(iz + 1);
+      } else {
+        const cx<T>* sa[6];
+        const cx<T>* sb[6];
+        level_srcs(iz, sa, sb);
+        sa[2] = sa[5];
+        sb[2] = sb[5];
+        synth_pairs<T, NX, 3>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      }
       up = ra[0]; vp = rb[0];
       wp = ra[1];
       if (rb[1] < emin) mark_clamp(a.dirty, clampF(iz + 1));
@@ -969,12 +1068,17 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows(RowArgs<T> a) {
       ex = ra[2]; ey = rb[2];
     } else {
       T ra[6], rb[6];
-      const cx<T>* sa[6];
-      const cx<T>* sb[6];
       const bool da[6] = {false, false, true, true, true, true};
       const bool db[6] = {false, false, false, false, false, false};
-      level_srcs(iz, sa, sb);
-      synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      if constexpr (PRE) {
+        synth_pre<T, NXB, MX, F::TPW, 6>(buf, P, da, db, ra, rb, a.kxp, tw, tid);
+        if (iz + 1 < z1) prefetch(iz + 1);
+      } else {
+        const cx<T>* sa[6];
+        const cx<T>* sb[6];
+        level_srcs(iz, sa, sb);
+        synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      }
       up = ra[0]; vp = rb[0];
       wp = ra[1];
       if (rb[1] < emin) mark_clamp(a.dirty, clampF(iz + 1));
