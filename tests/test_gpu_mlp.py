@@ -19,6 +19,8 @@ through the C-ABI.
   sgs_mlp.LearnedClosureSmall, plus the stage-1 SGS mean profiles; and that
   mode's two-stream stage against its one-stream stage (bit-identical).
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -184,18 +186,25 @@ def test_learned_stage_matches_oracle(kind, nz, stages):
     net.close()
 
 
-@pytest.mark.parametrize("kind,nz,stages", [("reject", 8, (1, 2, 3)), ("mixed", 8, (1, 2, 3)),
-                                            ("mixed", 64, (1,))])
-def test_small_grid_closure_matches_oracle(kind, nz, stages):
+@pytest.mark.parametrize("kind,n,nz,stages", [("reject", 128, 8, (1, 2, 3)), ("mixed", 128, 8, (1, 2, 3)),
+                                              ("mixed", 128, 64, (1,)), ("mixed", 512, 4, (1,))])
+def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
     """The closure evaluated on the nnx x nny grid (CF_CLOSURE_GRID_SMALL)
-    against sgs_mlp.LearnedClosureSmall, with the stage-1 SGS means."""
+    against sgs_mlp.LearnedClosureSmall, with the stage-1 SGS means.  512^2
+    exercises the eight-wavefront row kernel.  Among its 1M cells a few lie
+    on the acceptance boundary within BF16 rounding, and one that flips
+    between GPU and oracle (the counts still agree) moves u, v, w near it by
+    up to the closure's own local effect.  So there u, v, w are held to 1 %
+    of the closure's effect on the stage (the oracle's learned stage against
+    its TKE stage) instead of 1e-6; away from two such cells the fields
+    agree to 3e-5 of that effect, as at 128^2 and 256^2."""
     import channelflow as C
-    pr0 = O.Params(128, 128, nz, dt=0.002)
+    pr0 = O.Params(n, n, nz, dt=0.002)
     u, v, w, e = O.initial_fields(pr0)
     flat, norm = _closure_net(kind)
     onet = M.Network(flat, *[np.asarray(a, float) for a in norm], bf16_mode=True)
     cl = M.LearnedClosureSmall(onet)
-    pr = O.Params(128, 128, nz, dt=0.002, closure=cl)
+    pr = O.Params(n, n, nz, dt=0.002, closure=cl)
     st = O.set_state(pr, u, v, w, e)
     g = _les(pr, C.CF_PREC_FP64)
     g.set_state(u, v, w, e)
@@ -205,6 +214,7 @@ def test_small_grid_closure_matches_oracle(kind, nz, stages):
     g.closure_counts()
     for istage in stages:
         mn = istage == 1
+        st_tke = copy.deepcopy(st) if n != 128 else None
         O.stage(st, pr, istage, mnout=mn)
         g.stage(istage, mnout=mn)
         acc, tot = g.closure_counts()
@@ -212,13 +222,19 @@ def test_small_grid_closure_matches_oracle(kind, nz, stages):
         gs = g.get_state()
         errs = {k: rel(gs[k], O.interior(st, pr, k)) for k in "uvwe"}
         print("small", kind, istage, acc, ref_acc, tot, cl.last_accept.size, errs, flush=True)
+        tol = {k: 1e-10 if kind == "reject" else 1e-6 for k in "uvwe"}
+        if st_tke is not None and kind != "reject":
+            O.stage(st_tke, pr0, istage, mnout=mn)
+            effect = {k: rel(O.interior(st, pr, k), O.interior(st_tke, pr0, k)) for k in "uvw"}
+            print("  closure effect", effect, flush=True)
+            tol.update({k: 1e-2 * effect[k] for k in "uvw"})
+            tol["e"] = 1e-10
         # divz comes from the band spectra on this path (k_divz_band)
         assert np.allclose(g.divz(), st.divz, rtol=1e-8, atol=1e-25)
         assert tot == cl.last_accept.size
         assert abs(acc - ref_acc) <= max(2, 1e-3 * tot)
-        tol = 1e-10 if kind == "reject" else 1e-6
         for k in "uvwe":
-            assert errs[k] < tol, (istage, k, errs)
+            assert errs[k] < tol[k], (istage, k, errs, tol)
         if kind == "reject":
             assert acc == 0
         else:
