@@ -43,6 +43,9 @@ struct ColShape {
   static constexpr int BYTES = 4 * NY * (int)sizeof(cx<T>);  // per column of the tile
   static constexpr int TK0 = 65536 / BYTES;
   static constexpr int TK = TK0 < MX ? TK0 : MX;             // columns per workgroup
+  // k_col_synth / k_col_anal: at least one transform per wavefront of the
+  // 512-thread workgroup (they pass one batch of TKC transforms through LDS)
+  static constexpr int TKC = TK >= 8 ? TK : (MX < 8 ? MX : 8);
   // Eight waves per tile.  The column kernels pass one batch of TK padded
   // transforms through LDS at a time (about 32 KB at 256^2 FP32), so four
   // workgroups share a CU.
@@ -355,7 +358,7 @@ template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
-  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
+  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TKC, NT = Sh::NT;
   constexpr int PLB = padlen(NYB), PLS = padlen(NY);
   constexpr int LPS = (TK * NY + NT - 1) / NT;        // band values per thread
   constexpr int NZP = NYB - NY + 1;                    // zero modes NY/2 .. NYB - NY/2
@@ -2025,7 +2028,7 @@ template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
-  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TK, NT = Sh::NT;
+  constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TKC, NT = Sh::NT;
   constexpr int PLB = padlen(NYB), PLS = padlen(NY);
   constexpr int NKT = MX / TK;
   constexpr int LPT = (TK * NYB + NT - 1) / NT;      // tile elements per thread

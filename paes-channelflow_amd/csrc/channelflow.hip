@@ -301,7 +301,7 @@ struct EngineT : Engine {
       a.twNYB = cptr(ctx->twNXB);
       a.nz = nz;
       a.lean = small ? 1 : 0;
-      hipLaunchKernelGGL((cf::k_col_synth<T, NX, NY>), dim3(MX / CS::TK, nz, 4), dim3(CS::NT), 0,
+      hipLaunchKernelGGL((cf::k_col_synth<T, NX, NY>), dim3(MX / CS::TKC, nz, 4), dim3(CS::NT), 0,
                          s, a);
     }
     if (overlap) {                 // forked after k_col_synth: the closure chain overlaps k_rows
@@ -399,7 +399,7 @@ struct EngineT : Engine {
       a.twNY = cptr(ctx->twNX);
       a.twNYB = cptr(ctx->twNXB);
       a.nz = nz;
-      const int ntiles = (MX / CS::TK) * nz * 4;      // tiles: kx fastest, then level, then component
+      const int ntiles = (MX / CS::TKC) * nz * 4;     // tiles: kx fastest, then level, then component
       hipLaunchKernelGGL((cf::k_col_anal<T, NX, NY>), dim3(ntiles), dim3(CS::NT), 0, s, a);
     }
     mark(true);
