@@ -52,6 +52,15 @@ struct ColShape {
   static constexpr int NT = 512;
 };
 
+// Column tile of workgroup x among n tiles of one plane.  Workgroups go to the
+// eight XCDs round-robin (x mod 8), and tiles 2j, 2j+1 share every 128-byte
+// line of the rows they read (each covers half).  For n a multiple of 16 the
+// two go to workgroups x and x + 8: the same XCD, so the second half-line
+// read hits that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int x, int n) {
+  return (n % 16 == 0) ? ((x & ~15) | ((x & 7) << 1) | ((x >> 3) & 1)) : x;
+}
+
 template <typename T, int NX>
 struct RowShape {
   static constexpr int MX = NX / 2;
@@ -366,7 +375,7 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   const int f = blockIdx.z;
   const int z = blockIdx.y + 1;
   if (f >= 2 && z >= a.nz) return;       // w, e live on faces 1..nz-1
-  const int k0 = blockIdx.x * TK;
+  const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK;
   const int tid = threadIdx.x;
   __shared__ cx<T> buf[TK * PLB];
   __shared__ cx<T> twb[NYB];
@@ -1927,7 +1936,7 @@ __global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
   constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = padlen(NY);
   const int f = blockIdx.z, z = blockIdx.y + 1;
   if (f == 2 && z >= a.nz) return;                 // w lives on faces 1..nz-1
-  const int k0 = blockIdx.x * TK, tid = threadIdx.x;
+  const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK, tid = threadIdx.x;
   __shared__ cx<T> buf[2 * TK * PL];
   __shared__ cx<T> tw[NY];
   load_twiddles(tw, a.twNY, NY, tid, NT);
@@ -1973,7 +1982,7 @@ __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
   constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = padlen(NY);
   const int cmp = blockIdx.z, z = blockIdx.y + 1;
   if (cmp == 2 && z >= a.nz) return;               // rw(nnz) = 0
-  const int k0 = blockIdx.x * TK, tid = threadIdx.x;
+  const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK, tid = threadIdx.x;
   __shared__ cx<T> buf[2 * TK * PL];
   __shared__ cx<T> tw[NY];
   load_twiddles(tw, a.twNY, NY, tid, NT);
@@ -2043,7 +2052,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
   const size_t splane = (size_t)NY * MX;
   const int t = blockIdx.x;
   if (t >= NKT * nz * 4) return;
-  const int kt = t % NKT, rest = t / NKT;
+  const int kt = xcd_tile(t % NKT, NKT), rest = t / NKT;
   const int z = rest % nz + 1, cmp = rest / nz;
   const int k0 = kt * TK;
   if (cmp >= 2 && z == nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
@@ -2491,7 +2500,7 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
   __shared__ cx<T> tw[NY];
   const int tid = threadIdx.x;
   load_twiddles(tw, twg, NY, tid, NT);
-  const int k0 = blockIdx.x * TK;
+  const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK;
   const int L = l0 + blockIdx.y;
   const cx<T>* src = in + (size_t)L * NY * MX + k0;
 #pragma unroll
