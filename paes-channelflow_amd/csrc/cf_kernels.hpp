@@ -516,13 +516,16 @@ __device__ __forceinline__ void fill_pair(cx<T>* z, const cx<T>* sa, bool da, co
 // Synthesise NP (<= 6) real row pairs; this thread's point of the two real
 // rows of pair q comes back in ra[q], rb[q].  Ends with a workgroup barrier
 // after which the caller must not write buf until the next anal_pairs.
-template <typename T, int NX, int NP>
+// WV: each wavefront owns whole transforms (needs NXB a multiple of 64);
+// otherwise the workgroup shares every transform.
+template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE>
 __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, const bool* da,
                                             const cx<T>* const* sb, const bool* db, T* ra, T* rb,
                                             const T* __restrict__ kxp, const cx<T>* tw, int tid) {
   using F = RowFFT<T, NX>;
   constexpr int NXB = F::NXB;
-  if constexpr (F::WAVE) {
+  static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
+  if constexpr (WV) {
     const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
     for (int qq = 0; qq < F::TPW; ++qq) {
@@ -559,11 +562,12 @@ __device__ __forceinline__ V pick(const V* arr, int q, V dflt) {
 // Forward-transform NP real row pairs (this thread holds point tid of each)
 // and hand coefficient k < MX of both members, scaled by 1/NXB, to
 // emit(q, k, A, B).
-template <typename T, int NX, int NP, typename Emit>
+template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE, typename Emit>
 __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb, const cx<T>* tw,
                                            int tid, Emit emit) {
   using F = RowFFT<T, NX>;
   constexpr int MX = F::MX, NXB = F::NXB;
+  static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
   const T s = T(0.5) / T(NXB);
   __syncthreads();                       // every thread has read the synthesis results
 #pragma unroll
@@ -574,7 +578,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     const cx<T> zn = buf[q * padlen(NXB) + pidx(k == 0 ? 0 : NXB - k)];
     emit(q, k, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s}, cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
   };
-  if constexpr (F::WAVE) {
+  if constexpr (WV) {
     const int w = tid >> 6, lane = tid & 63;
     if (w * F::TPW < NP) {
       fft_run<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw, lane);
@@ -954,6 +958,11 @@ template <typename T, int NX, bool LEAN = false>
 __global__ void __launch_bounds__(3 * NX / 2) __attribute__((amdgpu_waves_per_eu(rows_prefetch<T, NX>() ? 4 : 1)))
 k_rows(RowArgs<T> a) {
   using F = RowFFT<T, NX>;
+  // Wave-owned transforms, except for the FP32 lean sweep at nnx >= 512: there
+  // the twelve wavefronts of a row share every transform (three synthesis
+  // pairs would leave three quarters of them idle; measured 7.3 -> 6.1 ms per
+  // stage at 512^2 x 256).  The other nnx = 512 kernels spill more that way.
+  constexpr bool WV = F::WAVE && (NX <= 256 || !(LEAN && sizeof(T) == 4));
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
   constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
   __shared__ cx<T> buf[6 * padlen(NXB)];
@@ -1007,7 +1016,7 @@ k_rows(RowArgs<T> a) {
     sb[3] = row(FE, clampF(L1));
     sa[4] = (L0 >= 1 && L0 <= nz - 1) ? row(FW, L0) : nullptr;
     sb[4] = (L0 >= 1 && L0 <= nz - 1) ? row(FWY, L0) : nullptr;
-    synth_pairs<T, NX, 5>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+    synth_pairs<T, NX, 5, WV>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
     um = ra[0]; vm = rb[0];
     u0 = ra[1]; v0 = rb[1];
     wm = ra[2]; w0 = ra[3];
@@ -1041,7 +1050,7 @@ k_rows(RowArgs<T> a) {
     sb[5] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
   };
   // wave-owned row transforms take their half spectra a level ahead
-  constexpr bool PRE = F::WAVE && rows_prefetch<T, NX>();
+  constexpr bool PRE = WV && rows_prefetch<T, NX>();
   constexpr int NPS = LEAN ? 3 : 6;                 // synthesis pairs per level
   RowPre<T, MX, F::TPW> P;
   auto prefetch = [&](int iz) {
@@ -1070,7 +1079,7 @@ k_rows(RowArgs<T> a) {
         level_srcs(iz, sa, sb);
         sa[2] = sa[5];
         sb[2] = sb[5];
-        synth_pairs<T, NX, 3>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+        synth_pairs<T, NX, 3, WV>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
       }
       up = ra[0]; vp = rb[0];
       wp = ra[1];
@@ -1089,7 +1098,7 @@ k_rows(RowArgs<T> a) {
         const cx<T>* sa[6];
         const cx<T>* sb[6];
         level_srcs(iz, sa, sb);
-        synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+        synth_pairs<T, NX, 6, WV>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
       }
       up = ra[0]; vp = rb[0];
       wp = ra[1];
@@ -1172,7 +1181,7 @@ k_rows(RowArgs<T> a) {
     }
     // partial sums of the SGS stress means, when means are due (slot 0, divz,
     // stays unused: k_divz_band computes it from the band spectra)
-    if constexpr (F::WAVE) {
+    if constexpr (WV) {
       const int nq = mn ? 7 : 0;
       for (int q = 1; q < nq; ++q) {
 #pragma unroll
@@ -1204,9 +1213,9 @@ k_rows(RowArgs<T> a) {
           default: if (!top) A[7 * as + o + k] = fA; break;                    // Fye
         }
       };
-      anal_pairs<T, NX, 6>(buf, fa, fb, tw, tid, emit);
+      anal_pairs<T, NX, 6, WV>(buf, fa, fb, tw, tid, emit);
     }
-    if constexpr (F::WAVE) {
+    if constexpr (WV) {
       // wave partials were published by the barriers inside anal_pairs
       const int nq = mn ? 7 : 0;
       if (tid >= 1 && tid < nq) {

@@ -256,6 +256,35 @@ def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
     net.close()
 
 
+@pytest.mark.parametrize("n", [256, 512])
+def test_fp32_small_grid_stage_close_to_fp64(n):
+    """The FP32 small-grid closure stage (lean 3/2-grid sweep: wave-owned
+    transforms at 256^2, workgroup-shared at 512^2) against the FP64 one,
+    which the oracle pins, from the same state.  With a network that every
+    cell rejects no acceptance decision can flip, so the fields differ by FP32
+    rounding alone: within 2e-5 relative L2, as the FP32 TKE step."""
+    import channelflow as C
+    pr = O.Params(n, n, 4, dt=0.002)
+    u, v, w, e = O.initial_fields(pr)
+    flat, norm = _closure_net("reject")
+    net = make_net(flat, norm)
+    out = {}
+    for prec in (C.CF_PREC_FP64, C.CF_PREC_FP32):
+        g = _les(pr, prec)
+        g.set_state(u, v, w, e)
+        g.set_closure(net)
+        g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL)
+        g.stage(1)
+        out[prec] = g.get_state()
+        assert g.closure_counts()[0] == 0
+        g.close()
+    errs = {k: rel(out[C.CF_PREC_FP32][k], out[C.CF_PREC_FP64][k]) for k in "uvwe"}
+    print("fp32 vs fp64 small-grid stage", n, errs, flush=True)
+    for k in "uvwe":
+        assert errs[k] < 2e-5, (k, errs)
+    net.close()
+
+
 def test_two_stream_stage_equals_one_stream_stage():
     """In the nnx x nny mode the closure's divz, columns-in and row kernels run
     on a second stream beside the 3/2-grid sweep; the per-kernel timing pass of
