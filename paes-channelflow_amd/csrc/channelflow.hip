@@ -52,7 +52,8 @@ struct Engine;
 
 struct cf_ctx {
   cf_params p{};
-  int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16, zchunk_mlp = 32, ncu = 256;
+  int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16, zchunk_mlp = 32, zchunk_sgs = 32,
+      ncu = 256;
   bool fp32 = false;
   hipStream_t stream = nullptr;
   std::string err;
@@ -569,8 +570,8 @@ struct EngineT : Engine {
     a.slots = (int)S;
     a.counts = ctx->closure_counts;
     a.c = c;
-    a.c.zchunk = ctx->zchunk_mlp;
-    const int nch = (nz + ctx->zchunk_mlp - 1) / ctx->zchunk_mlp;
+    a.c.zchunk = ctx->zchunk_sgs;
+    const int nch = (nz + ctx->zchunk_sgs - 1) / ctx->zchunk_sgs;
     hipLaunchKernelGGL((cf::k_sgs_rows<T, NX>), dim3(NY, nch), dim3(NX), 0, s, a, ctx->closure_net);
     HIPCHK(hipGetLastError());
     return CF_OK;
@@ -763,6 +764,9 @@ int cf_create(const cf_params* p, cf_ctx** out) {
   ctx->dt = p->dt;
   ctx->zchunk = nz >= 64 ? 16 : 8;
   ctx->zchunk_mlp = nz >= 64 ? 32 : ctx->zchunk;
+  // k_sgs_rows: each chunk evaluates the network at one extra level; 64-level
+  // chunks halve that (measured 0.826 -> 0.810 ms per stage at 256^2 x 128)
+  ctx->zchunk_sgs = nz >= 128 ? 64 : ctx->zchunk_mlp;
   if (hipSetDevice(p->device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "hipSetDevice/hipStreamCreate failed";
     delete ctx;
