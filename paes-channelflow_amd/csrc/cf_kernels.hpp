@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(256) k_wall_final(const double* __restrict__ p
 
 // ---------------------------------------------------------------------------
 // k_col_synth: band spectrum -> 3/2-grid x-spectral rows, value and d/dy
-// grid (MX/TK, nz, 4 fields u,v,w,e)
+// grid (MX/TKC, nz, 4 fields: e, w, v, u in blockIdx.z order)
 //
 // A workgroup holds TK band columns of one level and field in registers and
 // passes the value, then the y-derivative, through one batch of TK
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   constexpr int LPS = (TK * NY + NT - 1) / NT;        // band values per thread
   constexpr int NZP = NYB - NY + 1;                    // zero modes NY/2 .. NYB - NY/2
   static_assert(PLS <= PLB, "the length-NY batch fits the length-NYB batch");
-  const int f = blockIdx.z;
+  const int f = 3 - (int)blockIdx.z;     // e first: its tiles carry three transforms, u, v, w one or two
   const int z = blockIdx.y + 1;
   if (f >= 2 && z >= a.nz) return;       // w, e live on faces 1..nz-1
   const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK;
@@ -2022,7 +2022,7 @@ __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
 // ---------------------------------------------------------------------------
 // k_col_anal: y-forward of the flux rows, band truncation, R = G + i ky Fy.
 // One tile per workgroup: TK kx columns of one level and component; tiles
-// kx fastest, then level, then component (u, v, w, e).
+// kx fastest, then level, then component (e, w, v, u: the heaviest first).
 //
 // G and F^y pass one after the other through one batch of TK transforms in
 // LDS, so that four workgroups share a CU; both tiles are loaded into
@@ -2062,7 +2062,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
   const int t = blockIdx.x;
   if (t >= NKT * nz * 4) return;
   const int kt = xcd_tile(t % NKT, NKT), rest = t / NKT;
-  const int z = rest % nz + 1, cmp = rest / nz;
+  const int z = rest % nz + 1, cmp = 3 - rest / nz;    // e first: three transforms, u, v, w two
   const int k0 = kt * TK;
   if (cmp >= 2 && z == nz) {                 // rw(nnz) = re(nnz) = 0 (comp1.f90:174-177)
     cx<T>* o = (cmp == 2 ? a.R[2] : a.Tre) + (size_t)z * splane + k0;
