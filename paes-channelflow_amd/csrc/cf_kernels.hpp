@@ -53,9 +53,10 @@ struct ColShape {
 };
 
 // Column tile of workgroup x among n tiles of one plane.  Workgroups go to the
-// eight XCDs round-robin (x mod 8), and tiles 2j, 2j+1 share every 128-byte
-// line of the rows they read (each covers half).  For n a multiple of 16 the
-// two go to workgroups x and x + 8: the same XCD, so the second half-line
+// eight XCDs round-robin by linear workgroup id; x varies fastest and n is a
+// multiple of 16 wherever this remaps, so the XCD is x mod 8.  Tiles 2j and
+// 2j+1 share every 128-byte line of the rows they read (each covers half);
+// they go to workgroups x and x + 8, the same XCD, so the second half-line
 // read hits that XCD's L2.
 __device__ __forceinline__ int xcd_tile(int x, int n) {
   return (n % 16 == 0) ? ((x & ~15) | ((x & 7) << 1) | ((x >> 3) & 1)) : x;
