@@ -256,15 +256,20 @@ __device__ __forceinline__ void wall_tau(const WallScalars* ws, int wall, T u, T
 }
 
 // ---------------------------------------------------------------------------
-// k_wall: wall-model scalars.  One workgroup of 64 threads.
+// k_wall: wall-model scalars.  One workgroup of 64 threads.  As the stage's
+// first kernel it also clears the stage's clamp marks (dirty[ndirty]) and the
+// divz accumulators (red0[nred0]), in place of two memset dispatches.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
                                              const cx<T>* __restrict__ Ruo,
                                              const cx<T>* __restrict__ Rvo, size_t plane, Consts c,
-                                             WallScalars* ws) {
+                                             WallScalars* ws, int* dirty, int ndirty, double* red0,
+                                             int nred0) {
   const int nz = c.nz;
   const int tid = threadIdx.x;
+  for (int i = tid; i < ndirty; i += 64) dirty[i] = 0;
+  for (int i = tid; i < nred0; i += 64) red0[i] = 0.0;
   // dudt, dvdt: volume means of the previous stage's R (lowerbc.f90:70-72);
   // the plane sum of a band field is nx*ny times its (0,0) coefficient
   double su = 0.0, sv = 0.0;

@@ -284,10 +284,10 @@ struct EngineT : Engine {
     bool small = false;
     if constexpr (NX % 64 == 0) small = ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL;
     const bool overlap = small && !ev;
-    HIPCHK(hipMemsetAsync(ctx->dirty, 0, ((size_t)nz + 3) * sizeof(int), s));
     mark(false);
     hipLaunchKernelGGL(cf::k_wall<T>, dim3(1), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
-                       cptr(ctx->Ruo), cptr(ctx->Rvo), band_plane, c, ctx->ws);
+                       cptr(ctx->Ruo), cptr(ctx->Rvo), band_plane, c, ctx->ws, ctx->dirty, nz + 3,
+                       ctx->red, ctx->slots);
     mark(true);
     {
       cf::SynthArgs<T> a{};
@@ -525,7 +525,7 @@ struct EngineT : Engine {
   // a band-limited field is resolved there, so the two agree to rounding);
   // into the q = 0 slots of red.
   int divz_band(cf_ctx* ctx, double dzm, hipStream_t s) {
-    HIPCHK(hipMemsetAsync(ctx->red, 0, (size_t)ctx->slots * sizeof(double), s));   // the q = 0 slots
+    // the q = 0 slots of red were cleared by the stage's k_wall
     const int parts = std::max(1, std::min(16, (MX * NY) / 2048));
     hipLaunchKernelGGL(cf::k_divz_band<T>, dim3(ctx->nz, parts), dim3(256), 0, s, cptr(ctx->U), cptr(ctx->V),
                        cptr(ctx->W), rptr(ctx->kxp), rptr(ctx->kyp), MX, NY, ctx->nz, dzm, ctx->red);
