@@ -52,3 +52,16 @@ def test_product_has_no_oracle_dependency():
             if f.endswith((".py", ".hip", ".hpp", ".h", ".F90", ".f90")):
                 txt = open(os.path.join(dirpath, f), errors="ignore").read()
                 assert "cf_oracle" not in txt and "oracle/" not in txt, f
+
+
+def test_kernel_families_of_the_timing_api():
+    """cf_time_steps reports CF_NKERN kernel families; the header's constant,
+    the binding's and the library's names agree (host calls only)."""
+    import channelflow as C
+    lib = C.load_library()
+    m = re.search(r"#define\s+CF_NKERN\s+(\d+)", open(HEADER).read())
+    assert m and int(m.group(1)) == C.CF_NKERN
+    names = [lib.cf_kernel_name(k).decode() for k in range(C.CF_NKERN)]
+    assert names == ["k_wall", "k_col_synth", "k_divz_band", "k_rows", "k_col_anal", "k_sgs_cols_in",
+                     "k_sgs_rows", "k_sgs_cols_out", "k_press", "k_erows", "k_reduce_part"]
+    assert lib.cf_kernel_name(C.CF_NKERN).decode() == "" and lib.cf_kernel_name(-1).decode() == ""
