@@ -216,8 +216,8 @@ def cpu_baseline(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU z-slab design is described, not built)"}))
+        print(json.dumps({"error": "the timestep runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU z-slab decomposition is designed, not built)"}))
         sys.exit(2)
     pts = args.nx * args.nx * args.nz
     if args.copy_probe:
