@@ -138,6 +138,27 @@ def test_fp32_wave_path_step_close_to_oracle():
     g.close()
 
 
+@pytest.mark.parametrize("n,nz", [(128, 16), (512, 4)])
+def test_fp32_step_close_to_fp64_step(n, nz):
+    """The FP32 TKE path against the FP64 path, which the oracle pins, from the
+    same state, at the sizes whose FP32 kernels no other test runs: 128^2
+    (three-wavefront rows) and 512^2 (twelve-wavefront rows, eight-column
+    column tiles).  One RK3 step, within 5e-5 relative L2, as at 256^2."""
+    import channelflow as C
+    pr = O.Params(n, n, nz, dt=0.002)
+    u, v, w, e = O.initial_fields(pr)
+    out = {}
+    for prec in (C.CF_PREC_FP64, C.CF_PREC_FP32):
+        g = make(pr, prec)
+        g.set_state(u, v, w, e)
+        g.step(1)
+        out[prec] = g.get_state()
+        g.close()
+    for k in "uvwe":
+        err = rel(out[C.CF_PREC_FP32][k], out[C.CF_PREC_FP64][k])
+        assert err < 5e-5, (n, k, err)
+
+
 def test_fp64_128x64_fields_divergence_means_spectra():
     """The 128^2 x 64 channel (Re_tau 180, dt = 1/256): three steps on the GPU
     and in the oracle.  Fields within 1e-10, the small-grid divergence at the
