@@ -85,21 +85,23 @@ NML_LEARNED = """&RESTARTING restart=.false., restart_file='start.field' /
 &TIMING case='chan', timax=1, savfrq=1, meanfrq=1, isbgr=.true., dt={dt}, imodel=3, isurface=1 /
 &GRIDSIZE nnx=128, nny=128, nnz=16 /
 &PARAMS xl=6.283185307179586, yl=3.141592653589793, zl=2.0, z0=3.4262e-4 /
-&GPU precision=1, seed=10972, amp=0.1, closure=1, weights='{weights}' /
+&GPU precision=1, seed=10972, amp=0.1, closure=1, weights='{weights}', closure_grid={grid} /
 """
 
 
-def test_driver_learned_closure_matches_the_library_run(tmp_path):
+@pytest.mark.parametrize("grid", [0, 1])
+def test_driver_learned_closure_matches_the_library_run(tmp_path, grid):
     """closure=1 in the GPU namelist group loads the shipped network and runs
-    the FP32 path with it; the .field output equals the same run made through
-    the C-ABI from Python."""
+    the FP32 path with it, on the 3/2 grid (closure_grid=0) or the nnx x nny
+    grid (1); the .field output equals the same run made through the C-ABI
+    from Python."""
     import channelflow as C
     from channelflow.fieldio import read_field
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built")
     weights = os.path.join(ROOT, "paes-channelflow_amd", "weights", "sgs_mlp.cfmlp")
     dt = 1.0 / 512                      # CFL ~0.3 on the 192-point 3/2 grid
-    (tmp_path / "nlist.input").write_text(NML_LEARNED.format(dt=dt, weights=weights))
+    (tmp_path / "nlist.input").write_text(NML_LEARNED.format(dt=dt, weights=weights, grid=grid))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Learned SGS closure" in r.stdout and "Learned closure used at" in r.stdout
@@ -108,6 +110,8 @@ def test_driver_learned_closure_matches_the_library_run(tmp_path):
     g.seeded_init(10972, 0.1)
     net = C.SgsNet(0, weights)
     g.set_closure(net)
+    if grid == 1:
+        g.set_closure_grid(C.CF_CLOSURE_GRID_SMALL)
     g.step(512)
     s = g.get_state()
     acc, tot = g.closure_counts()
