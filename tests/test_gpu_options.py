@@ -5,7 +5,8 @@ Both row-transform paths: 32^2 (3/2-grid rows of 48 points shared by the
 workgroup) and 128^2 (rows of 192 points owned by wavefronts).  Bound:
 relative L2 <= 1e-10 on u, v, w, e after every stage, p <= 1e-8, utau to
 1e-12.  The oracle's wall models are pinned against the compiled reference
-(tests/test_oracle_ref.py) and by invariants (tests/test_oracle_wall.py)."""
+(tests/test_oracle_ref.py) and by invariants (tests/test_oracle_wall.py).
+The FP32 path with each option is held to 1e-4 of the FP64 path at 256^2."""
 import numpy as np
 import pytest
 
@@ -43,3 +44,26 @@ def test_wall_and_sgs_models_match_oracle(isurface, imodel, n, nz):
             assert errs["p"] < 1e-8, (step, istage, errs)
             assert abs(g.scalars()["utau"] - st.utau) < 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("isurface,imodel", [(2, 3), (3, 3), (4, 3), (5, 3), (6, 3), (1, 4)])
+def test_fp32_options_close_to_fp64(isurface, imodel):
+    """The FP32 path with each wall model and with imodel 4, against the FP64
+    path (pinned above), at 256^2 (the FP32 row kernel that loads its spectra
+    a level ahead): one RK3 step from the same state, within 1e-4 relative
+    L2."""
+    import channelflow as C
+    pr = O.Params(256, 256, 8, dt=0.002, isurface=isurface, imodel=imodel)
+    u, v, w, e = O.initial_fields(pr)
+    out = {}
+    for prec in (C.CF_PREC_FP64, C.CF_PREC_FP32):
+        g = C.ChannelFlow(C.RunParams(256, 256, 8, xl=pr.xl, yl=pr.yl, zl=pr.zl, z0=pr.z0, dt=pr.dt,
+                                      imodel=imodel, isurface=isurface, precision=prec))
+        g.set_state(u, v, w, e)
+        g.step(1)
+        out[prec] = g.get_state()
+        g.close()
+    errs = {k: rel(out[C.CF_PREC_FP32][k], out[C.CF_PREC_FP64][k]) for k in "uvwe"}
+    print("fp32 vs fp64", isurface, imodel, {k: f"{x:.1e}" for k, x in errs.items()}, flush=True)
+    for k in "uvwe":
+        assert errs[k] < 1e-4, (k, errs)
