@@ -1359,24 +1359,44 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
     ok0 = closure(zs, g, t0);
   }
 
+  // the six synthesis pairs of iteration iz: u, v at iz+2; w, e at face
+  // iz+1; the horizontal gradients of u, v at iz+1 and of w at face iz+1;
+  // ex, ey at face iz (at iz = nnz-1 face nnz-2 again, derivs2.f90:279)
+  auto level_srcs = [&](int iz, const cx<T>** sa, const cx<T>** sb) {
+    sa[0] = lvl(FU, iz + 2);  sb[0] = lvl(FV, iz + 2);
+    sa[1] = face(FW, iz + 1); sb[1] = row(FE, clampF(iz + 1));
+    sa[2] = lvl(FU, iz + 1);  sb[2] = lvl(FUY, iz + 1);
+    sa[3] = lvl(FV, iz + 1);  sb[3] = lvl(FVY, iz + 1);
+    sa[4] = face(FW, iz + 1); sb[4] = face(FWY, iz + 1);
+    const int ie = (iz == nz - 1) ? nz - 2 : iz;
+    sa[5] = (iz >= 1 && iz <= nz - 1) ? row(FE, ie) : nullptr;
+    sb[5] = (iz >= 1 && iz <= nz - 1) ? row(FEY, ie) : nullptr;
+  };
+  // the half spectra a level ahead, as in k_rows
+  constexpr bool PRE = rows_prefetch<T, NX>();
+  RowPre<T, MX, F::TPW> P;
+  auto prefetch = [&](int iz) {
+    const cx<T>* sa[6];
+    const cx<T>* sb[6];
+    level_srcs(iz, sa, sb);
+    row_prefetch<T, MX, F::TPW, 6>(P, sa, sb, tid);
+  };
+  if constexpr (PRE) prefetch(zs);
   for (int iz = zs; iz < z1; ++iz) {
     T ex, ey;
     {
       T ra[6], rb[6];
-      const cx<T>* sa[6];
-      const cx<T>* sb[6];
       const bool da[6] = {false, false, true, true, true, true};
       const bool db[6] = {false, false, false, false, false, false};
-      sa[0] = lvl(FU, iz + 2);  sb[0] = lvl(FV, iz + 2);
-      sa[1] = face(FW, iz + 1); sb[1] = row(FE, clampF(iz + 1));
-      sa[2] = lvl(FU, iz + 1);  sb[2] = lvl(FUY, iz + 1);
-      sa[3] = lvl(FV, iz + 1);  sb[3] = lvl(FVY, iz + 1);
-      sa[4] = face(FW, iz + 1); sb[4] = face(FWY, iz + 1);
-      // ex, ey at face iz; at iz = nnz-1 face nnz-2 again (derivs2.f90:279)
-      const int ie = (iz == nz - 1) ? nz - 2 : iz;
-      sa[5] = (iz >= 1 && iz <= nz - 1) ? row(FE, ie) : nullptr;
-      sb[5] = (iz >= 1 && iz <= nz - 1) ? row(FEY, ie) : nullptr;
-      synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      if constexpr (PRE) {
+        synth_pre<T, NXB, MX, F::TPW, 6>(buf, P, da, db, ra, rb, a.kxp, tw, tid);
+        if (iz + 1 < z1) prefetch(iz + 1);
+      } else {
+        const cx<T>* sa[6];
+        const cx<T>* sb[6];
+        level_srcs(iz, sa, sb);
+        synth_pairs<T, NX, 6>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+      }
       upp = ra[0]; vpp = rb[0];
       wp = ra[1];
       if (rb[1] < emin) mark_clamp(a.dirty, clampF(iz + 1));
