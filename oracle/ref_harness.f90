@@ -1,5 +1,5 @@
 ! ref_harness.f90 -- test driver (ours) around the reference's own FFT-free
-! subroutines, compiled from /root/reference by oracle/build_ref.sh in FP64
+! subroutines, compiled from the reference's sources by oracle/build_ref.sh in FP64
 ! (-fdefault-real-8 -fdefault-double-8).  TEST INFRASTRUCTURE ONLY.
 !
 ! mode 1: feeds the 3/2-grid gradients, expanded level-1/nnz velocities and

@@ -1,7 +1,7 @@
 """Pins the oracle's SGS/wall-model layer and tridiagonal solver against the
 reference's own Fortran (complen.f90, compsgs1.f90, lowerbc.f90, upperbc.f90,
-tridv.f90, compiled in FP64 from /root/reference by oracle/build_ref.sh into
-oracle/_ref).  The reference's FFT-dependent files need FFTW 2.x, which is not
+tridv.f90, compiled in FP64 from the reference's sources by oracle/build_ref.sh
+into oracle/_ref).  The reference's FFT-dependent files need FFTW 2.x, which is not
 in this image, so only this FFT-free subset is built.  Skipped when neither the
 binary nor the reference sources are present."""
 import os
@@ -20,11 +20,9 @@ BUILD = os.path.join(ROOT, "oracle", "build_ref.sh")
 @pytest.fixture(scope="module")
 def harness():
     if not os.path.exists(HARNESS):
-        if not os.path.isdir("/root/reference"):
-            pytest.skip("reference sources absent and oracle/_ref not built")
         r = subprocess.run(["sh", BUILD], capture_output=True, text=True)
-        if r.returncode != 0:
-            pytest.skip("oracle/_ref build failed: " + r.stderr[-500:])
+        if r.returncode != 0 or not os.path.exists(HARNESS):
+            pytest.skip("oracle/_ref not built (reference sources absent?): " + r.stderr[-500:])
     return HARNESS
 
 
