@@ -21,8 +21,8 @@ its mean launch time from HIP events on the library stream, against the 8 TB/s
 HBM3E peak.  ``step_roofline`` prices the whole step at the compulsory bytes of
 the persistent arrays (FP32 path: 85 FP32 + 12 FP64 array passes = 436 B per
 grid-point step; FP64 path: 97 x 8 = 776 B).  ``cpu_baseline`` times the FP64
-numpy oracle (the CPU restatement of the reference stage) on one stage of the
-same 256x256x128 workload, on one host core.
+numpy oracle (the CPU restatement of the reference stage) on one whole RK3
+step of the same 256x256x128 workload, on one host core.
 """
 from __future__ import annotations
 
@@ -197,19 +197,20 @@ def seeded_fields(nx, nz, z0):
 
 
 def cpu_baseline(args):
-    """FP64 numpy oracle (the CPU restatement) on one RK3 stage of the same
-    grid; reported in grid-point RK3 steps/s = points / (3 * stage time)."""
+    """FP64 numpy oracle (the CPU restatement) on one whole RK3 step (three
+    stages) of the same grid, about 15 s on one core at 256^2 x 128; reported
+    in grid-point RK3 steps/s = points / step time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cf_oracle as O
     nz = args.cpu_sample_nz or args.nz
     pr = O.Params(args.nx, args.nx, nz, z0=math.exp(-2.0) / args.re_tau, dt=0.002)
     st = O.set_state(pr, *O.initial_fields(pr))
     t0 = time.perf_counter()
-    O.stage(st, pr, 1)
+    O.step(st, pr)
     t = time.perf_counter() - t0
     pts = args.nx * args.nx * nz
-    return {"value": pts / (3.0 * t), "unit": "grid-pt RK3 steps/s", "cores": 1, "kind": "port",
-            "sample": f"one RK3 stage (x3 for a step) of the FP64 numpy oracle at "
+    return {"value": pts / t, "unit": "grid-pt RK3 steps/s", "cores": 1, "kind": "port",
+            "sample": f"one RK3 step (three stages) of the FP64 numpy oracle at "
                       f"{args.nx}x{args.nx}x{nz}, {t:.1f} s"}
 
 
@@ -320,7 +321,7 @@ def main():
             "finite": r64["finite"],
         }
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (numpy oracle, one stage)", file=sys.stderr, flush=True)
+        print("[bench] cpu baseline (numpy oracle, one step)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args)
     else:
         line["cpu_baseline"] = None
