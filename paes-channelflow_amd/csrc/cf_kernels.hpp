@@ -161,6 +161,7 @@ struct RedSize {
 // (lowerbc.f90:121).
 // ---------------------------------------------------------------------------
 constexpr double BETAA = 0.35;                                 // parameters.f90:8
+constexpr double FORCING = 0.016;                              // imposed dp/dx, comp1.f90:152
 
 __device__ inline void wall_coeffs(WallScalars* ws, const Consts& c, int wall, const double* st,
                                    double N) {
@@ -183,7 +184,7 @@ __device__ inline void wall_coeffs(WallScalars* ws, const Consts& c, int wall, c
       return;
     }
     case 6: {                                                   // mean-forcing balance
-      const double a6u = -c.zl / 2.0 * (0.016 - ws->dudt), a6v = -c.zl / 2.0 * (-ws->dvdt);
+      const double a6u = -c.zl / 2.0 * (FORCING - ws->dudt), a6v = -c.zl / 2.0 * (-ws->dvdt);
       const double r = sg * utau * ws->utau_s / vsfc;
       A[0] = -sg * a6u + r * dU; A[1] = r;
       B[0] = -sg * a6v + r * dV; B[1] = r;
@@ -260,10 +261,11 @@ __device__ __forceinline__ void wall_tau(const WallScalars* ws, int wall, T u, T
 // first kernel it also clears the stage's clamp marks (dirty[ndirty]) and the
 // divz accumulators (red0[nred0]), in place of two memset dispatches.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
-                                             const cx<T>* __restrict__ Ruo,
-                                             const cx<T>* __restrict__ Rvo, size_t plane, Consts c,
+// M0 holds the FP64 (0,0) coefficients, i.e. the plane means, of u, v, ru,
+// rv: M0[q * slots + level] for q = 0..3 (M0_U .. M0_RV).
+enum : int { M0_U = 0, M0_V = 1, M0_RU = 2, M0_RV = 3 };
+
+__global__ void __launch_bounds__(64) k_wall(const double* __restrict__ M0, int slots, Consts c,
                                              WallScalars* ws, int* dirty, int ndirty, double* red0,
                                              int nred0) {
   const int nz = c.nz;
@@ -272,10 +274,12 @@ __global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const 
   for (int i = tid; i < nred0; i += 64) red0[i] = 0.0;
   // dudt, dvdt: volume means of the previous stage's R (lowerbc.f90:70-72);
   // the plane sum of a band field is nx*ny times its (0,0) coefficient
+  const double* mu = M0 + M0_U * slots;
+  const double* mv = M0 + M0_V * slots;
   double su = 0.0, sv = 0.0;
   for (int z = 1 + tid; z <= nz; z += 64) {
-    su += (double)Ruo[(size_t)z * plane].x;
-    sv += (double)Rvo[(size_t)z * plane].x;
+    su += M0[M0_RU * slots + z];
+    sv += M0[M0_RV * slots + z];
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -284,14 +288,14 @@ __global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const 
   }
   if (tid != 0) return;
   // plane means = the (0,0) coefficient (lowerbc.f90:57-58)
-  double u1 = (double)U[1 * plane].x, v1 = (double)V[1 * plane].x;
+  double u1 = mu[1], v1 = mv[1];
   double vs = sqrt(u1 * u1 + v1 * v1);
   vs = vs > 0.01 ? vs : 0.01;                                   // lowerbc.f90:65
   double ut = vs * c.vk / c.zody;                               // lowerbc.f90:67
   ws->utau_lo = ut;
   ws->c_lo = ut * ut / vs;
-  double un = (double)U[(size_t)nz * plane].x + c.ugal;        // upperbc.f90:241-242
-  double vn = (double)V[(size_t)nz * plane].x + c.vgal;
+  double un = mu[nz] + c.ugal;                                  // upperbc.f90:64-65
+  double vn = mv[nz] + c.vgal;
   double vs2 = sqrt(un * un + vn * vn);
   vs2 = vs2 > 0.01 ? vs2 : 0.01;
   double ut2 = vs2 * c.vk / c.zody;
@@ -300,11 +304,11 @@ __global__ void __launch_bounds__(64) k_wall(const cx<T>* __restrict__ U, const 
   ws->utau = ut2;                                               // sufsim_upper runs last
   ws->dudt = su / nz;
   ws->dvdt = sv / nz;
-  ws->utau_s = sqrt(c.zl / 2.0 * sqrt(ws->dvdt * ws->dvdt + (0.016 - ws->dudt) * (0.016 - ws->dudt)));
+  ws->utau_s = sqrt(c.zl / 2.0 * sqrt(ws->dvdt * ws->dvdt + (FORCING - ws->dudt) * (FORCING - ws->dudt)));
   if (!(ut <= 10.0) || !(ut2 <= 10.0)) ws->flags |= FLAG_UTAU;  // lowerbc.f90:77-80
   ws->mu[0] = u1; ws->mv[0] = v1;
-  ws->mu[1] = (double)U[(size_t)nz * plane].x;
-  ws->mv[1] = (double)V[(size_t)nz * plane].x;
+  ws->mu[1] = mu[nz];
+  ws->mv[1] = mv[nz];
   ws->vsfc[0] = vs; ws->vsfc[1] = vs2;
   ws->wind_terms = (c.isurface == 2 || c.isurface == 4 || c.isurface == 5) ? 1 : 0;
   if (c.isurface == 1 || c.isurface == 6) {                   // no plane statistics needed
@@ -1160,7 +1164,8 @@ k_rows(RowArgs<T> a) {
       }
       // compadv1.f90:19-51 and compsgs3.f90:205-224
       fa[0] = -(u0 * u0 + tuu);
-      fb[0] = T(0.016) - ((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
+      // the imposed gradient 0.016 (comp1.f90:152) is a (0,0)-mode constant: k_press adds it in FP64
+      fb[0] = -((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
       fa[1] = -(u0 * v0 + tuv);
       fb[1] = -((vp + v0) * w0 - (vm + v0) * wm) * todzm - (tvw0 - tvwm) * dzm;
       fa[3] = -(v0 * v0 + tvv);
@@ -1456,7 +1461,8 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
         }
         // compadv1.f90:19-51 and compsgs3.f90:205-224
         fa[0] = -(u0 * u0 + tuu);
-        fb[0] = T(0.016) - ((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
+        // the imposed gradient 0.016 (comp1.f90:152) is a (0,0)-mode constant: k_press adds it in FP64
+        fb[0] = -((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
         fa[1] = -(u0 * v0 + tuv);
         fb[1] = -((vp + v0) * w0 - (vm + v0) * wm) * todzm - (tvw0 - tvwm) * dzm;
         fa[3] = -(v0 * v0 + tvv);
@@ -2180,49 +2186,127 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // k_press: substep + Poisson RHS + tridiagonal z-solve (FP64) + projection and
-// RK update.  One thread per band mode (kx, j).  grid ceil(NY*MX/64)
+// RK update (comprhs.f90:1-37, press.F90:1-115, tridv.f90:1-55, comp2.f90:1-45).
+//
+// The z operator of a mode (kx, ky) has real coefficients (sub = super = 1,
+// diag = -dz^2 k^2 - 2, -1 at the Neumann rows 1 and nnz), so the real and the
+// imaginary part of the mode are two independent real systems: lane 2m solves
+// the real part of mode m, lane 2m+1 the imaginary part.  A lane loads both
+// parts of its mode's inputs in the forward sweep (the pair's 8- or 16-byte
+// loads fall on the same addresses), because i kx f mixes them, and only its
+// own part in the backward sweep; the projection takes the partner's p
+// through a lane swap.
+//
+// The reciprocal pivots d_z depend on k^2 and dz only: the forward sweep
+// recomputes them and the backward sweep reads the identical values from the
+// table k_press_pivots writes once (piv[slot][mode]).
+//
+// Mode (0,0) is left to the grid's last workgroup: its pressure is pinned to
+// 0 (press.F90:81-88), so its levels are independent.  Its u, v, ru, rv live
+// in FP64 in M0 (the band holds them rounded to T), and the imposed pressure
+// gradient 0.016 (comp1.f90:152) is added to its ru there, in FP64.
+// grid ceil(2 NY MX / 64) + 1 workgroups of 64.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PressArgs {
   cx<T>* U[3];            // U, V, W band spectra (in: stage start, out: new)
   cx<T>* Ro[3];           // R registers (in: previous stage final, out: this stage final)
-  const cx<T>* Rn[3];     // comp1 tendencies of this stage
+  const cx<T>* Rn[3];     // comp1 tendencies of this stage (without the 0.016 forcing)
   cx<double>* P;          // pressure band spectrum (out)
-  double* sd;             // scratch: reciprocal pivots [slots][modes]
-  cx<double>* sr;         // scratch: forward-eliminated rhs [slots][modes]
+  const double* piv;      // reciprocal pivots [slots][modes] (k_press_pivots)
+  double* sr;             // scratch: forward-eliminated rhs [slots][2 modes], one part per lane
+  double* M0;             // FP64 (0,0) coefficients [4][slots]: u, v, ru, rv
   const double* kxd;      // kx wavenumbers (MX)
   const double* kyd;      // ky wavenumbers per band row (NY)
-  int mx, ny, nz, istage;
+  int mx, ny, nz, istage, slots;
   double dz, dzm, dtab1, dtab2, dtabm;
 };
 
+// Reciprocal pivots of the Thomas elimination (tridv.f90:28-43) for every
+// mode and level: d_1 = 1/diag_1, d_z = 1/(diag_z - d_z-1).  One thread per mode.
+__global__ void __launch_bounds__(64) k_press_pivots(double* piv, const double* kxd, const double* kyd,
+                                                     int mx, int ny, int nz, double dz) {
+  const int modes = mx * ny;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= modes) return;
+  const int j = m / mx, kx = m - j * mx;
+  const double kxp = kxd[kx], kyp = kyd[j];
+  const double dz2 = dz * dz;
+  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:37-43)
+  double d_prev = 0.0;
+  for (int z = 1; z <= nz; ++z) {
+    const double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
+    const double d = (z == 1) ? 1.0 / diag : 1.0 / (diag - d_prev);
+    piv[(size_t)z * modes + m] = d;
+    d_prev = d;
+  }
+}
+
 template <typename T>
-__device__ __forceinline__ cx<double> ld2(const cx<T>* p) {
-  const cx<T> v = *p;
-  return cx<double>{(double)v.x, (double)v.y};
+__device__ __forceinline__ double part_of(const cx<T>& v, bool im) {
+  return im ? (double)v.y : (double)v.x;
 }
 template <typename T>
-__device__ __forceinline__ void st2(cx<T>* p, cx<double> v) {
-  *p = cx<T>{(T)v.x, (T)v.y};
+__device__ __forceinline__ void store_part(cx<T>* p, bool im, double v) {
+  T* q = reinterpret_cast<T*>(p) + (im ? 1 : 0);
+  *q = (T)v;
+}
+
+// The mode (0,0): substep, the forcing, the RK update in FP64; w, rw, p = 0.
+template <typename T>
+__device__ void press_zero_mode(const PressArgs<T>& a) {
+  const size_t pl = (size_t)a.mx * a.ny;
+  double* mu = a.M0 + M0_U * a.slots;
+  double* mv = a.M0 + M0_V * a.slots;
+  double* mru = a.M0 + M0_RU * a.slots;
+  double* mrv = a.M0 + M0_RV * a.slots;
+  const bool sub = (a.istage != 1);
+  for (int z = 1 + threadIdx.x; z <= a.nz; z += blockDim.x) {
+    const size_t o = (size_t)z * pl;
+    double u = mu[z], v = mv[z];
+    if (sub) {                                   // comp1.f90:126-128
+      u -= a.dtab2 * mru[z];
+      v -= a.dtab2 * mrv[z];
+    }
+    const double ru = (double)a.Rn[0][o].x + FORCING;   // comp1.f90:152
+    const double rv = (double)a.Rn[1][o].x;
+    u += a.dtab1 * ru;                           // comp2.f90:35-36
+    v += a.dtab1 * rv;
+    mu[z] = u; mv[z] = v; mru[z] = ru; mrv[z] = rv;
+    a.U[0][o] = cx<T>{(T)u, T(0)};
+    a.U[1][o] = cx<T>{(T)v, T(0)};
+    a.U[2][o] = cx<T>{T(0), T(0)};               // the w and rw plane means are removed (comp2.f90:30,39)
+    a.Ro[0][o] = cx<T>{(T)ru, T(0)};
+    a.Ro[1][o] = cx<T>{(T)rv, T(0)};
+    a.Ro[2][o] = cx<T>{T(0), T(0)};
+    a.P[o] = cx<double>{0.0, 0.0};
+  }
 }
 
 // The z sweeps are latency bound (one dependent step per level), so each
-// thread loads the inputs of ZU levels before running their ZU recurrence
-// steps: ZU levels of loads are in flight per memory round trip.  FP32
-// inputs take ZU = 8 (218 VGPRs; 512 wavefronts at 256^2 leave the register
-// file free); FP64 inputs stay at 4 (191 VGPRs).
+// lane loads the inputs of ZU levels before running their ZU recurrence
+// steps: ZU levels of loads are in flight per memory round trip.
 template <typename T>
 __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   constexpr int ZU = sizeof(T) == 4 ? 8 : 4;
+  if (blockIdx.x == gridDim.x - 1) {
+    press_zero_mode(a);
+    return;
+  }
   const int modes = a.mx * a.ny;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= modes) return;
-  const int j = m / a.mx, kx = m - j * a.mx;
-  if (j == a.ny / 2) return;                    // Nyquist row: always zero
-  const bool zero_mode = (j == 0 && kx == 0);
+  const int lane_id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = lane_id >> 1;
+  const bool im = (lane_id & 1) != 0;
+  // (lanes past the last mode and the Nyquist row keep running the shuffles
+  // of the backward sweep with their results discarded: every lane of the
+  // wave takes part in __shfl_xor)
+  const bool live = m < modes;
+  const int mm = live ? m : 0;
+  const int j = mm / a.mx, kx = mm - j * a.mx;
+  const bool active = live && j != a.ny / 2 && mm != 0;   // Nyquist row: always zero; (0,0): last workgroup
   const double kxp = a.kxd[kx], kyp = a.kyd[j];
   const double dz2 = a.dz * a.dz;
-  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:446-455)
+  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:37-43)
   const int nz = a.nz;
   const size_t pl = (size_t)modes;
   const bool sub = (a.istage != 1);
@@ -2235,124 +2319,144 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   const cx<T>* __restrict__ N0 = a.Rn[0];
   const cx<T>* __restrict__ N1 = a.Rn[1];
   const cx<T>* __restrict__ N2 = a.Rn[2];
-  double* __restrict__ sd = a.sd;
-  cx<double>* __restrict__ sr = a.sr;
-  // forward sweep (comprhs.f90:17-35, press.F90:87-133, tridv.f90:200-209)
-  cx<double> wm_prev = {0, 0}, rw_prev = {0, 0}, r_prev = {0, 0};
-  double d_prev = 0.0;
-  for (int zb = 1; zb <= nz; zb += ZU) {
-    cx<T> lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU], lou[ZU], lov[ZU], low[ZU];
+  double* __restrict__ sr = a.sr;
+  // the partner part of i k f: (i k f).re = -k f.im, (i k f).im = k f.re
+  const double sgn = im ? 1.0 : -1.0;
+  if (active) {
+    // forward sweep (comprhs.f90:17-35, press.F90:44-76, tridv.f90:28-43)
+    double wm_prev = 0.0, rw_prev = 0.0, r_prev = 0.0, d_prev = 0.0;
+    for (int zb = 1; zb <= nz; zb += ZU) {
+      cx<T> lu[ZU], lv[ZU], lru[ZU], lrv[ZU], lou[ZU], lov[ZU];
+      T lw[ZU], lrw[ZU], low[ZU];
 #pragma unroll
-    for (int q = 0; q < ZU; ++q) {
-      const int z = zb + q;
-      if (z <= nz) {
+      for (int q = 0; q < ZU; ++q) {
+        const int z = zb + q;
+        if (z <= nz) {
+          const size_t o = (size_t)z * pl + m;
+          lu[q] = U0[o]; lv[q] = U1[o];
+          lw[q] = reinterpret_cast<const T*>(U2 + o)[im];
+          lru[q] = N0[o]; lrv[q] = N1[o];
+          lrw[q] = reinterpret_cast<const T*>(N2 + o)[im];
+          if (sub) {
+            lou[q] = R0[o]; lov[q] = R1[o];
+            low[q] = reinterpret_cast<const T*>(R2 + o)[im];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < ZU; ++q) {
+        const int z = zb + q;
+        if (z > nz) break;
         const size_t o = (size_t)z * pl + m;
-        lu[q] = U0[o]; lv[q] = U1[o]; lw[q] = U2[o];
-        lru[q] = N0[o]; lrv[q] = N1[o]; lrw[q] = N2[o];
-        if (sub) { lou[q] = R0[o]; lov[q] = R1[o]; low[q] = R2[o]; }
+        double ur = (double)lu[q].x, ui = (double)lu[q].y;
+        double vr = (double)lv[q].x, vi = (double)lv[q].y;
+        double w = (double)lw[q];
+        if (sub) {                                 // comp1.f90:126-133
+          ur -= (double)lou[q].x * a.dtab2; ui -= (double)lou[q].y * a.dtab2;
+          vr -= (double)lov[q].x * a.dtab2; vi -= (double)lov[q].y * a.dtab2;
+          if (z != nz) w -= (double)low[q] * a.dtab2;
+          store_part(U0 + o, im, im ? ui : ur);
+          store_part(U1 + o, im, im ? vi : vr);
+          store_part(U2 + o, im, w);
+        }
+        // the partner part of f = R + phi / (gamma dt)
+        const double fx = (im ? (double)lru[q].x : (double)lru[q].y) + (im ? ur : ui) * a.dtabm;
+        const double fy = (im ? (double)lrv[q].x : (double)lrv[q].y) + (im ? vr : vi) * a.dtabm;
+        const double rw = (double)lrw[q];
+        double rhs = sgn * (fx * kxp) + sgn * (fy * kyp);
+        rhs += ((w - wm_prev) * a.dtabm + (rw - rw_prev)) * a.dzm;
+        wm_prev = w;
+        rw_prev = rw;
+        const double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
+        double r = rhs * dz2;
+        double d;
+        if (z == 1) {
+          d = 1.0 / diag;
+        } else {
+          d = 1.0 / (diag - d_prev);               // a(i-1) = b(i) = 1
+          r -= r_prev * d_prev;
+        }
+        sr[(size_t)z * 2 * pl + lane_id] = r;
+        d_prev = d;
+        r_prev = r;
       }
-    }
-#pragma unroll
-    for (int q = 0; q < ZU; ++q) {
-      const int z = zb + q;
-      if (z > nz) break;
-      const size_t o = (size_t)z * pl + m;
-      cx<double> u = {(double)lu[q].x, (double)lu[q].y};
-      cx<double> v = {(double)lv[q].x, (double)lv[q].y};
-      cx<double> w = {(double)lw[q].x, (double)lw[q].y};
-      if (sub) {                                 // comp1.f90:126-133
-        u = csub(u, cscale(cx<double>{(double)lou[q].x, (double)lou[q].y}, a.dtab2));
-        v = csub(v, cscale(cx<double>{(double)lov[q].x, (double)lov[q].y}, a.dtab2));
-        if (z != nz) w = csub(w, cscale(cx<double>{(double)low[q].x, (double)low[q].y}, a.dtab2));
-        st2(U0 + o, u);
-        st2(U1 + o, v);
-        st2(U2 + o, w);
-      }
-      const cx<double> ru = {(double)lru[q].x, (double)lru[q].y};
-      const cx<double> rv = {(double)lrv[q].x, (double)lrv[q].y};
-      const cx<double> rw = {(double)lrw[q].x, (double)lrw[q].y};
-      const cx<double> fx = cadd(ru, cscale(u, a.dtabm));
-      const cx<double> fy = cadd(rv, cscale(v, a.dtabm));
-      cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
-      const cx<double> dw = cadd(cscale(csub(w, wm_prev), a.dtabm), csub(rw, rw_prev));
-      rhs = cadd(rhs, cscale(dw, a.dzm));
-      wm_prev = w;
-      rw_prev = rw;
-      double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
-      double lo = 1.0;
-      cx<double> r = cscale(rhs, dz2);
-      if (zero_mode) {                           // press.F90:125-132
-        diag = 1.0;
-        lo = 0.0;
-        r = cx<double>{0, 0};
-      }
-      double d;
-      if (z == 1) {
-        d = 1.0 / diag;
-      } else {
-        const double fac = lo * d_prev;
-        d = 1.0 / (diag - fac * lo);             // a(i-1) = 1 (0 for the zero mode)
-        r = csub(r, cscale(r_prev, fac));
-      }
-      sd[o] = d;
-      sr[o] = r;
-      d_prev = d;
-      r_prev = r;
     }
   }
-  // backward sweep (tridv.f90:210-219) fused with comp2 (comp2.f90:238-263)
-  const double up_c = zero_mode ? 0.0 : 1.0;
-  cx<double> p_next = {0, 0};
+  // backward sweep (tridv.f90:44-53) fused with comp2 (comp2.f90:21-41)
+  double p_next = 0.0;
   for (int zt = nz; zt >= 1; zt -= ZU) {
-    double ld[ZU];
-    cx<double> lr[ZU];
-    cx<T> lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU];
+    double ld[ZU], lr[ZU];
+    T lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU];
+    if (active) {
 #pragma unroll
-    for (int q = 0; q < ZU; ++q) {
-      const int z = zt - q;
-      if (z >= 1) {
-        const size_t o = (size_t)z * pl + m;
-        ld[q] = sd[o]; lr[q] = sr[o];
-        lu[q] = U0[o]; lv[q] = U1[o]; lw[q] = U2[o];
-        lru[q] = N0[o]; lrv[q] = N1[o]; lrw[q] = N2[o];
+      for (int q = 0; q < ZU; ++q) {
+        const int z = zt - q;
+        if (z >= 1) {
+          const size_t o = (size_t)z * pl + m;
+          ld[q] = a.piv[o];
+          lr[q] = sr[(size_t)z * 2 * pl + lane_id];
+          lu[q] = reinterpret_cast<const T*>(U0 + o)[im];
+          lv[q] = reinterpret_cast<const T*>(U1 + o)[im];
+          lw[q] = reinterpret_cast<const T*>(U2 + o)[im];
+          lru[q] = reinterpret_cast<const T*>(N0 + o)[im];
+          lrv[q] = reinterpret_cast<const T*>(N1 + o)[im];
+          lrw[q] = reinterpret_cast<const T*>(N2 + o)[im];
+        }
       }
     }
 #pragma unroll
     for (int q = 0; q < ZU; ++q) {
       const int z = zt - q;
       if (z < 1) break;
-      const size_t o = (size_t)z * pl + m;
-      const double d = ld[q];
-      const cx<double> r = lr[q];
-      const cx<double> u = {(double)lu[q].x, (double)lu[q].y};
-      const cx<double> v = {(double)lv[q].x, (double)lv[q].y};
-      const cx<double> w = {(double)lw[q].x, (double)lw[q].y};
-      cx<double> ru = {(double)lru[q].x, (double)lru[q].y};
-      cx<double> rv = {(double)lrv[q].x, (double)lrv[q].y};
-      cx<double> rw = {(double)lrw[q].x, (double)lrw[q].y};
-      cx<double> p = (z == nz) ? cscale(r, d) : cscale(csub(r, cscale(p_next, up_c)), d);
-      if (zero_mode) p = cx<double>{0, 0};       // press.F90:151-152
-      ru = csub(ru, cmuli(cscale(p, kxp)));      // comp2.f90:241-242
-      rv = csub(rv, cmuli(cscale(p, kyp)));      // comp2.f90:246-247
-      cx<double> wn = {0, 0};
-      if (z != nz) {
-        rw = csub(rw, cscale(csub(p_next, p), a.dzm));   // comp2.f90:252
-        if (zero_mode) rw = cx<double>{0, 0};            // comp2.f90:253
-        wn = cadd(w, cscale(rw, a.dtab1));               // comp2.f90:260
-        if (zero_mode) wn = cx<double>{0, 0};            // comp2.f90:262
-      } else {
-        rw = cx<double>{0, 0};
+      double p = 0.0;
+      if (active) p = (z == nz) ? lr[q] * ld[q] : (lr[q] - p_next) * ld[q];
+      const double pp = __shfl_xor(p, 1, 64);    // the partner part of p
+      if (active) {
+        const size_t o = (size_t)z * pl + m;
+        // -(i k p) in this lane's part: re: +k p.im, im: -k p.re
+        const double ru = (double)lru[q] - sgn * (pp * kxp);   // comp2.f90:21
+        const double rv = (double)lrv[q] - sgn * (pp * kyp);   // comp2.f90:25
+        double rw = 0.0, wn = 0.0;
+        if (z != nz) {
+          rw = (double)lrw[q] - (p_next - p) * a.dzm;          // comp2.f90:29
+          wn = (double)lw[q] + rw * a.dtab1;                   // comp2.f90:37
+        }
+        store_part(U0 + o, im, (double)lu[q] + ru * a.dtab1);  // comp2.f90:35-36
+        store_part(U1 + o, im, (double)lv[q] + rv * a.dtab1);
+        store_part(U2 + o, im, wn);
+        store_part(R0 + o, im, ru);
+        store_part(R1 + o, im, rv);
+        store_part(R2 + o, im, rw);
+        reinterpret_cast<double*>(a.P + o)[im] = p;
       }
-      st2(U0 + o, cadd(u, cscale(ru, a.dtab1)));        // comp2.f90:257-258
-      st2(U1 + o, cadd(v, cscale(rv, a.dtab1)));
-      st2(U2 + o, wn);
-      st2(R0 + o, ru);
-      st2(R1 + o, rv);
-      st2(R2 + o, rw);
-      a.P[o] = p;
       p_next = p;
     }
   }
+}
+
+// Load-time means: the band (0,0) coefficients of u and v take the FP64
+// plane means M0 (rounded to T, imaginary part 0); one thread per level.
+template <typename T>
+__global__ void k_set_mean(cx<T>* U, cx<T>* V, const double* __restrict__ M0, int slots, size_t plane,
+                           int nz) {
+  const int z = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (z > nz) return;
+  U[(size_t)z * plane] = cx<T>{(T)M0[M0_U * slots + z], T(0)};
+  V[(size_t)z * plane] = cx<T>{(T)M0[M0_V * slots + z], T(0)};
+}
+
+// Physical levels 1..nz in out (nxy points each) of a field synthesised from
+// a band whose (0,0) coefficient m0 holds in FP64: add m0[z] - band(0,0) and
+// the constant add to every point of level z.  grid (nz), 256 threads.
+template <typename T>
+__global__ void __launch_bounds__(256) k_shift_levels(double* out, const double* __restrict__ m0,
+                                                      const cx<T>* __restrict__ band, size_t plane,
+                                                      size_t nxy, double add) {
+  const int z = 1 + blockIdx.x;
+  double d = add;
+  if (m0) d += m0[z] - (double)band[(size_t)z * plane].x;
+  double* o = out + (size_t)(z - 1) * nxy;
+  for (size_t i = threadIdx.x; i < nxy; i += 256) o[i] += d;
 }
 
 // ---------------------------------------------------------------------------

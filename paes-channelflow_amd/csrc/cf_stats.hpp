@@ -24,9 +24,11 @@ constexpr int NMEAN = 10;   // uxym vxym wxym uu vv ww uw vw uv engsbz
 // - uu = <ul ul>, vv, ww = <w w>, uw = <ul w>, vw, uv;
 // - the level means of u, v, w;
 // - engsbz = <e>, from the physical e (which carries e(nnz) = e(nnz-1)).
+// The means of u and v come from their FP64 copies m0u, m0v (per level).
 template <typename T>
 __global__ void __launch_bounds__(256) k_compmn(const cx<T>* __restrict__ U, const cx<T>* __restrict__ V,
                                                 const cx<T>* __restrict__ W, const T* __restrict__ E,
+                                                const double* __restrict__ m0u, const double* __restrict__ m0v,
                                                 int mx, int ny, int nz, double* out) {
   __shared__ double red[6][256];
   const int iz = blockIdx.x + 1, tid = threadIdx.x;
@@ -78,8 +80,8 @@ __global__ void __launch_bounds__(256) k_compmn(const cx<T>* __restrict__ U, con
   }
   if (tid == 0) {
     double* o = out + (size_t)(iz - 1) * NMEAN;
-    o[0] = (double)u0[0].x;
-    o[1] = (double)v0[0].x;
+    o[0] = m0u[iz];
+    o[1] = m0v[iz];
     o[2] = (double)w0[0].x;
     for (int q = 0; q < 6; ++q) o[3 + q] = red[q][0];
     o[9] = esum[0] / (double)pplane;

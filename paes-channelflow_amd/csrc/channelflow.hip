@@ -71,8 +71,9 @@ struct cf_ctx {
   cx<double>* P = nullptr;
   cx<double>* xs64 = nullptr;
   cx<double>* tw64NX = nullptr;
-  double* sd = nullptr;
-  cx<double>* sr = nullptr;
+  double* piv = nullptr;        // k_press reciprocal pivots [slots][modes] (set at create)
+  double* sr = nullptr;         // k_press forward-eliminated rhs [slots][2 modes]
+  double* M0 = nullptr;         // FP64 (0,0) coefficients of u, v, ru, rv [4][slots]
   double* part = nullptr;
   double* red = nullptr;        // [7][slots]: divz and the 6 SGS means
   double* kxd = nullptr;
@@ -192,8 +193,9 @@ struct EngineT : Engine {
     if ((rc = dalloc(ctx, &ctx->P, band)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->xs64, band)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->tw64NX, (size_t)NX)) != CF_OK) return rc;
-    if ((rc = dalloc(ctx, &ctx->sd, S * NY * MX)) != CF_OK) return rc;
-    if ((rc = dalloc(ctx, &ctx->sr, S * NY * MX)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, &ctx->piv, S * NY * MX)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, &ctx->sr, 2 * S * NY * MX)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, &ctx->M0, 4 * S)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->part, 7 * S * NYB)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->red, 7 * S)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->kxd, (size_t)MX)) != CF_OK) return rc;
@@ -224,6 +226,11 @@ struct EngineT : Engine {
     HIPCHK(hipMemcpy(ctx->kyp, kyt.data(), NY * sizeof(T), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->kxd, kx.data(), MX * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->kyd, ky.data(), NY * sizeof(double), hipMemcpyHostToDevice));
+    // the pressure solve's reciprocal pivots depend on k^2 and dz only
+    hipLaunchKernelGGL(cf::k_press_pivots, dim3((MX * NY + 63) / 64), dim3(64), 0, ctx->stream, ctx->piv,
+                       ctx->kxd, ctx->kyd, MX, NY, ctx->nz, ctx->p.zl / ctx->nz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return CF_OK;
   }
 
@@ -267,7 +274,6 @@ struct EngineT : Engine {
     const int nz = ctx->nz;
     const bool sgs = mnout && istage == 1;
     const cf::Consts c = consts(ctx, istage, sgs ? 1 : 0);
-    const size_t band_plane = (size_t)NY * MX;
     const size_t bstride = (size_t)ctx->slots * NYB * MX;
     int ek = 0;
     // records the event that closes timing slot ek-1 (used: whether that slot
@@ -285,9 +291,8 @@ struct EngineT : Engine {
     if constexpr (NX % 64 == 0) small = ctx->closure_on && ctx->closure_grid == CF_CLOSURE_GRID_SMALL;
     const bool overlap = small && !ev;
     mark(false);
-    hipLaunchKernelGGL(cf::k_wall<T>, dim3(1), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
-                       cptr(ctx->Ruo), cptr(ctx->Rvo), band_plane, c, ctx->ws, ctx->dirty, nz + 3,
-                       ctx->red, ctx->slots);
+    hipLaunchKernelGGL(cf::k_wall, dim3(1), dim3(64), 0, s, ctx->M0, ctx->slots, c, ctx->ws, ctx->dirty,
+                       nz + 3, ctx->red, ctx->slots);
     mark(true);
     {
       cf::SynthArgs<T> a{};
@@ -441,8 +446,10 @@ struct EngineT : Engine {
       a.Rn[1] = cptr(ctx->Rvn);
       a.Rn[2] = cptr(ctx->Rwn);
       a.P = ctx->P;
-      a.sd = ctx->sd;
+      a.piv = ctx->piv;
       a.sr = ctx->sr;
+      a.M0 = ctx->M0;
+      a.slots = ctx->slots;
       a.kxd = ctx->kxd;
       a.kyd = ctx->kyd;
       a.mx = MX;
@@ -455,7 +462,8 @@ struct EngineT : Engine {
       a.dtab2 = c.dtab2;
       a.dtabm = c.dtabm;
       const int modes = MX * NY;
-      hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 63) / 64), dim3(64), 0, s, a);
+      // one lane per real and imaginary part, and one workgroup for mode (0,0)
+      hipLaunchKernelGGL(cf::k_press<T>, dim3((2 * modes + 63) / 64 + 1), dim3(64), 0, s, a);
     }
     mark(true);
     {
@@ -606,8 +614,26 @@ struct EngineT : Engine {
     if ((rc = to_band(ctx, u, cptr(ctx->U))) != CF_OK) return rc;
     if ((rc = to_band(ctx, v, cptr(ctx->V))) != CF_OK) return rc;
     if ((rc = to_band(ctx, w, cptr(ctx->W))) != CF_OK) return rc;
-    // w(nnz) is the top wall (upperbc.f90:203)
+    // w(nnz) is the top wall (upperbc.f90:27)
     HIPCHK(hipMemsetAsync(cptr(ctx->W) + (size_t)nz * NY * MX, 0, NY * MX * sizeof(C), s));
+    // the plane means of u and v in FP64 (the (0,0) coefficients; deal keeps
+    // them), the RK registers' means zero
+    std::vector<double> m0(4 * S, 0.0);
+    const size_t nxy = (size_t)NY * NX;
+    for (int z = 1; z <= nz; ++z) {
+      long double su = 0.0L, sv = 0.0L;
+      const double* uz = u + (size_t)(z - 1) * nxy;
+      const double* vz = v + (size_t)(z - 1) * nxy;
+      for (size_t i = 0; i < nxy; ++i) {
+        su += uz[i];
+        sv += vz[i];
+      }
+      m0[cf::M0_U * S + z] = (double)(su / (long double)nxy);
+      m0[cf::M0_V * S + z] = (double)(sv / (long double)nxy);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->M0, m0.data(), m0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((cf::k_set_mean<T>), dim3((nz + 63) / 64), dim3(64), 0, s, cptr(ctx->U), cptr(ctx->V),
+                       ctx->M0, (int)S, (size_t)NY * MX, nz);
     // e: deal in spectral space, back to physical, keep its x-spectral rows
     if ((rc = to_band(ctx, e, cptr(ctx->band_tmp))) != CF_OK) return rc;
     hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, true>), dim3(MX / CS::TK, nz), dim3(256), 0, s,
@@ -641,7 +667,8 @@ struct EngineT : Engine {
 
   int means(cf_ctx* ctx, double* out) override {
     hipLaunchKernelGGL((cf::k_compmn<T>), dim3(ctx->nz), dim3(256), 0, ctx->stream, cptr(ctx->U),
-                       cptr(ctx->V), cptr(ctx->W), rptr(ctx->e), MX, NY, ctx->nz, out);
+                       cptr(ctx->V), cptr(ctx->W), rptr(ctx->e), ctx->M0 + cf::M0_U * ctx->slots,
+                       ctx->M0 + cf::M0_V * ctx->slots, MX, NY, ctx->nz, out);
     HIPCHK(hipGetLastError());
     return CF_OK;
   }
@@ -679,6 +706,17 @@ struct EngineT : Engine {
       rc = from_band<double>(ctx, ctx->P, ctx->xs64, ctx->tw64NX);
     } else {
       rc = from_band<T>(ctx, cptr(bands[which]), cptr(ctx->xs_tmp), cptr(ctx->twNX));
+      // u, v, ru, rv: the plane means from their FP64 copies; the comp1
+      // tendency of u with the imposed gradient, which k_press adds (comp1.f90:152)
+      const int q = which == 0 ? cf::M0_U : which == 1 ? cf::M0_V : which == 5 ? cf::M0_RU
+                                                                   : which == 6 ? cf::M0_RV : -1;
+      const double add = which == 9 ? cf::FORCING : 0.0;
+      if (rc == CF_OK && (q >= 0 || add != 0.0)) {
+        hipLaunchKernelGGL((cf::k_shift_levels<T>), dim3(nz), dim3(256), 0, s, ctx->staging,
+                           q >= 0 ? ctx->M0 + (size_t)q * ctx->slots : nullptr, cptr(bands[which]),
+                           (size_t)NY * MX, (size_t)NY * NX, add);
+        HIPCHK(hipGetLastError());
+      }
     }
     return rc;
   }

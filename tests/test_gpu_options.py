@@ -3,7 +3,7 @@ on the GPU against the FP64 oracle, through the C-ABI.  Two full RK3 steps
 (six stages), so case 6 reads a nonzero previous-stage R for its dudt, dvdt.
 Both row-transform paths: 32^2 (3/2-grid rows of 48 points shared by the
 workgroup) and 128^2 (rows of 192 points owned by wavefronts).  Bound:
-relative L2 <= 1e-10 on u, v, w, e after every stage, p <= 1e-8, utau to
+relative L2 <= 1e-10 on u, v, w, e after every stage, p <= 1e-10, utau to
 1e-12.  The oracle's wall models are pinned against the compiled reference
 (tests/test_oracle_ref.py) and by invariants (tests/test_oracle_wall.py).
 The FP32 path with each option is held to 1e-4 of the FP64 path at 256^2."""
@@ -41,7 +41,7 @@ def test_wall_and_sgs_models_match_oracle(isurface, imodel, n, nz):
             print(isurface, imodel, n, step, istage, {k: f"{x:.1e}" for k, x in errs.items()}, flush=True)
             for k in "uvwe":
                 assert errs[k] < 1e-10, (step, istage, k, errs)
-            assert errs["p"] < 1e-8, (step, istage, errs)
+            assert errs["p"] < 1e-10, (step, istage, errs)
             assert abs(g.scalars()["utau"] - st.utau) < 1e-12
     g.close()
 

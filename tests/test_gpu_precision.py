@@ -136,9 +136,12 @@ def test_fp32_path_after_2000_steps_256():
             g.sync()
             print("precision", prec, "utau", g.scalars()["utau"], flush=True)
         h, _, _ = g.calc_spectra(zl)
-        out[prec] = (np.asarray(h), g.compmn())
+        out[prec] = (np.asarray(h), g.compmn(), g.get_state())
         g.close()
-    (h32, m32), (h64, m64) = out[C.CF_PREC_FP32], out[C.CF_PREC_FP64]
+    (h32, m32, s32), (h64, m64, s64) = out[C.CF_PREC_FP32], out[C.CF_PREC_FP64]
+    fe = {k: rel(s32[k], s64[k]) for k in "uvwep"}
+    print("fp32 vs fp64 after 2000 steps, fields:", {k: f"{v:.2e}" for k, v in fe.items()}, flush=True)
+    assert fe["u"] < 1e-3 and fe["e"] < 1e-3, fe
     r = O.MEAN_NAMES.index
     du = float(np.max(np.abs(m32[r("uxym")] - m64[r("uxym")]) / np.abs(m64[r("uxym")])))
     de = rel(m32[r("englez")], m64[r("englez")])
@@ -149,6 +152,41 @@ def test_fp32_path_after_2000_steps_256():
           flush=True)
     assert du < 1e-3 and de < 1e-3, (du, de)
     assert ds_q < 1e-2 and ds < 2e-2, (ds_q, ds)
+
+
+def test_fp32_mean_mode_is_kept_in_fp64():
+    """The FP32 path keeps the plane means of u, v and of their RK registers,
+    and adds the imposed gradient 0.016 (comp1.f90:152), in FP64
+    (press.F90:81-88 pins that mode's pressure).  On a horizontally uniform
+    log-law flow only the (0,0) mode evolves: forcing against the wall
+    stresses and the SGS z-flux.  Over 1000 steps the FP32 mean profile then
+    follows the FP64 path's to the FP32 rounding of its tendencies alone
+    (relative 1e-8 per stage at most), where FP32 storage of the profile
+    would add half an FP32 ulp (3e-8 relative) at every stage."""
+    import channelflow as C
+    n, nz = 64, 32
+    z0 = C.z0_of_retau(395.0)
+    dt = 0.3 / (3.2 * (3 * n / 2) / (2 * math.pi) + 0.5 / (2.0 / nz))
+    pr = O.Params(n, n, nz, z0=z0, dt=dt)
+    u, v, w, e = O.initial_fields(pr, amp=0.0)
+    prof = {}
+    for prec in (C.CF_PREC_FP64, C.CF_PREC_FP32):
+        g = C.ChannelFlow(C.RunParams(n, n, nz, z0=z0, dt=dt, precision=prec))
+        g.set_state(u, v, w, e)
+        g.step(1000)
+        g.sync()
+        prof[prec] = g.compmn()[O.MEAN_NAMES.index("uxym")]
+        s = g.get_state()
+        prof[(prec, "u")] = s["u"].mean(axis=(1, 2))
+        g.close()
+    d = float(np.max(np.abs(prof[C.CF_PREC_FP32] - prof[C.CF_PREC_FP64]) / np.abs(prof[C.CF_PREC_FP64])))
+    ds = float(np.max(np.abs(prof[(C.CF_PREC_FP32, "u")] - prof[(C.CF_PREC_FP64, "u")])
+                      / np.abs(prof[(C.CF_PREC_FP64, "u")])))
+    moved = float(np.max(np.abs(prof[C.CF_PREC_FP64] - u.mean(axis=(1, 2))) / np.abs(u.mean(axis=(1, 2)))))
+    print(f"uniform flow, 1000 steps: profile change {moved:.2e}; fp32 vs fp64 uxym {d:.2e}, "
+          f"downloaded u means {ds:.2e}", flush=True)
+    assert moved > 1e-4                        # the profile did evolve
+    assert d < 1e-6 and ds < 1e-6, (d, ds)
 
 
 def test_fp64_100_steps_32cubed_matches_oracle():
