@@ -2187,15 +2187,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
 // ---------------------------------------------------------------------------
 // k_press: substep + Poisson RHS + tridiagonal z-solve (FP64) + projection and
 // RK update (comprhs.f90:1-37, press.F90:1-115, tridv.f90:1-55, comp2.f90:1-45).
-//
-// The z operator of a mode (kx, ky) has real coefficients (sub = super = 1,
-// diag = -dz^2 k^2 - 2, -1 at the Neumann rows 1 and nnz), so the real and the
-// imaginary part of the mode are two independent real systems: lane 2m solves
-// the real part of mode m, lane 2m+1 the imaginary part.  A lane loads both
-// parts of its mode's inputs in the forward sweep (the pair's 8- or 16-byte
-// loads fall on the same addresses), because i kx f mixes them, and only its
-// own part in the backward sweep; the projection takes the partner's p
-// through a lane swap.
+// One lane per band mode (kx, j).
 //
 // The reciprocal pivots d_z depend on k^2 and dz only: the forward sweep
 // recomputes them and the backward sweep reads the identical values from the
@@ -2205,7 +2197,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
 // 0 (press.F90:81-88), so its levels are independent.  Its u, v, ru, rv live
 // in FP64 in M0 (the band holds them rounded to T), and the imposed pressure
 // gradient 0.016 (comp1.f90:152) is added to its ru there, in FP64.
-// grid ceil(2 NY MX / 64) + 1 workgroups of 64.
+// grid ceil(NY MX / 64) + 1 workgroups of 64.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PressArgs {
@@ -2214,7 +2206,7 @@ struct PressArgs {
   const cx<T>* Rn[3];     // comp1 tendencies of this stage (without the 0.016 forcing)
   cx<double>* P;          // pressure band spectrum (out)
   const double* piv;      // reciprocal pivots [slots][modes] (k_press_pivots)
-  double* sr;             // scratch: forward-eliminated rhs [slots][2 modes], one part per lane
+  cx<double>* sr;         // scratch: forward-eliminated rhs [slots][modes]
   double* M0;             // FP64 (0,0) coefficients [4][slots]: u, v, ru, rv
   const double* kxd;      // kx wavenumbers (MX)
   const double* kyd;      // ky wavenumbers per band row (NY)
@@ -2240,16 +2232,6 @@ __global__ void __launch_bounds__(64) k_press_pivots(double* piv, const double* 
     piv[(size_t)z * modes + m] = d;
     d_prev = d;
   }
-}
-
-template <typename T>
-__device__ __forceinline__ double part_of(const cx<T>& v, bool im) {
-  return im ? (double)v.y : (double)v.x;
-}
-template <typename T>
-__device__ __forceinline__ void store_part(cx<T>* p, bool im, double v) {
-  T* q = reinterpret_cast<T*>(p) + (im ? 1 : 0);
-  *q = (T)v;
 }
 
 // The mode (0,0): substep, the forcing, the RK update in FP64; w, rw, p = 0.
@@ -2283,9 +2265,19 @@ __device__ void press_zero_mode(const PressArgs<T>& a) {
   }
 }
 
+template <typename T>
+__device__ __forceinline__ cx<double> widen(const cx<T>& v) {
+  return cx<double>{(double)v.x, (double)v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st2(cx<T>* p, cx<double> v) {
+  *p = cx<T>{(T)v.x, (T)v.y};
+}
+
 // The z sweeps are latency bound (one dependent step per level), so each
-// lane loads the inputs of ZU levels before running their ZU recurrence
-// steps: ZU levels of loads are in flight per memory round trip.
+// thread loads the inputs of ZU levels before running their ZU recurrence
+// steps: ZU levels of loads are in flight per memory round trip.  FP32
+// inputs take ZU = 8, FP64 inputs 4.
 template <typename T>
 __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   constexpr int ZU = sizeof(T) == 4 ? 8 : 4;
@@ -2294,16 +2286,10 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
     return;
   }
   const int modes = a.mx * a.ny;
-  const int lane_id = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = lane_id >> 1;
-  const bool im = (lane_id & 1) != 0;
-  // (lanes past the last mode and the Nyquist row keep running the shuffles
-  // of the backward sweep with their results discarded: every lane of the
-  // wave takes part in __shfl_xor)
-  const bool live = m < modes;
-  const int mm = live ? m : 0;
-  const int j = mm / a.mx, kx = mm - j * a.mx;
-  const bool active = live && j != a.ny / 2 && mm != 0;   // Nyquist row: always zero; (0,0): last workgroup
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= modes || m == 0) return;             // (0,0): the last workgroup
+  const int j = m / a.mx, kx = m - j * a.mx;
+  if (j == a.ny / 2) return;                    // Nyquist row: always zero
   const double kxp = a.kxd[kx], kyp = a.kyd[j];
   const double dz2 = a.dz * a.dz;
   const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:37-43)
@@ -2319,116 +2305,94 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   const cx<T>* __restrict__ N0 = a.Rn[0];
   const cx<T>* __restrict__ N1 = a.Rn[1];
   const cx<T>* __restrict__ N2 = a.Rn[2];
-  double* __restrict__ sr = a.sr;
-  // the partner part of i k f: (i k f).re = -k f.im, (i k f).im = k f.re
-  const double sgn = im ? 1.0 : -1.0;
-  if (active) {
-    // forward sweep (comprhs.f90:17-35, press.F90:44-76, tridv.f90:28-43)
-    double wm_prev = 0.0, rw_prev = 0.0, r_prev = 0.0, d_prev = 0.0;
-    for (int zb = 1; zb <= nz; zb += ZU) {
-      cx<T> lu[ZU], lv[ZU], lru[ZU], lrv[ZU], lou[ZU], lov[ZU];
-      T lw[ZU], lrw[ZU], low[ZU];
+  cx<double>* __restrict__ sr = a.sr;
+  // forward sweep (comprhs.f90:17-35, press.F90:44-76, tridv.f90:28-43)
+  cx<double> wm_prev = {0, 0}, rw_prev = {0, 0}, r_prev = {0, 0};
+  double d_prev = 0.0;
+  for (int zb = 1; zb <= nz; zb += ZU) {
+    cx<T> lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU], lou[ZU], lov[ZU], low[ZU];
 #pragma unroll
-      for (int q = 0; q < ZU; ++q) {
-        const int z = zb + q;
-        if (z <= nz) {
-          const size_t o = (size_t)z * pl + m;
-          lu[q] = U0[o]; lv[q] = U1[o];
-          lw[q] = reinterpret_cast<const T*>(U2 + o)[im];
-          lru[q] = N0[o]; lrv[q] = N1[o];
-          lrw[q] = reinterpret_cast<const T*>(N2 + o)[im];
-          if (sub) {
-            lou[q] = R0[o]; lov[q] = R1[o];
-            low[q] = reinterpret_cast<const T*>(R2 + o)[im];
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < ZU; ++q) {
-        const int z = zb + q;
-        if (z > nz) break;
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zb + q;
+      if (z <= nz) {
         const size_t o = (size_t)z * pl + m;
-        double ur = (double)lu[q].x, ui = (double)lu[q].y;
-        double vr = (double)lv[q].x, vi = (double)lv[q].y;
-        double w = (double)lw[q];
-        if (sub) {                                 // comp1.f90:126-133
-          ur -= (double)lou[q].x * a.dtab2; ui -= (double)lou[q].y * a.dtab2;
-          vr -= (double)lov[q].x * a.dtab2; vi -= (double)lov[q].y * a.dtab2;
-          if (z != nz) w -= (double)low[q] * a.dtab2;
-          store_part(U0 + o, im, im ? ui : ur);
-          store_part(U1 + o, im, im ? vi : vr);
-          store_part(U2 + o, im, w);
-        }
-        // the partner part of f = R + phi / (gamma dt)
-        const double fx = (im ? (double)lru[q].x : (double)lru[q].y) + (im ? ur : ui) * a.dtabm;
-        const double fy = (im ? (double)lrv[q].x : (double)lrv[q].y) + (im ? vr : vi) * a.dtabm;
-        const double rw = (double)lrw[q];
-        double rhs = sgn * (fx * kxp) + sgn * (fy * kyp);
-        rhs += ((w - wm_prev) * a.dtabm + (rw - rw_prev)) * a.dzm;
-        wm_prev = w;
-        rw_prev = rw;
-        const double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
-        double r = rhs * dz2;
-        double d;
-        if (z == 1) {
-          d = 1.0 / diag;
-        } else {
-          d = 1.0 / (diag - d_prev);               // a(i-1) = b(i) = 1
-          r -= r_prev * d_prev;
-        }
-        sr[(size_t)z * 2 * pl + lane_id] = r;
-        d_prev = d;
-        r_prev = r;
+        lu[q] = U0[o]; lv[q] = U1[o]; lw[q] = U2[o];
+        lru[q] = N0[o]; lrv[q] = N1[o]; lrw[q] = N2[o];
+        if (sub) { lou[q] = R0[o]; lov[q] = R1[o]; low[q] = R2[o]; }
       }
+    }
+#pragma unroll
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zb + q;
+      if (z > nz) break;
+      const size_t o = (size_t)z * pl + m;
+      cx<double> u = widen(lu[q]), v = widen(lv[q]), w = widen(lw[q]);
+      if (sub) {                                 // comp1.f90:126-133
+        u = csub(u, cscale(widen(lou[q]), a.dtab2));
+        v = csub(v, cscale(widen(lov[q]), a.dtab2));
+        if (z != nz) w = csub(w, cscale(widen(low[q]), a.dtab2));
+        st2(U0 + o, u);
+        st2(U1 + o, v);
+        st2(U2 + o, w);
+      }
+      const cx<double> rw = widen(lrw[q]);
+      const cx<double> fx = cadd(widen(lru[q]), cscale(u, a.dtabm));
+      const cx<double> fy = cadd(widen(lrv[q]), cscale(v, a.dtabm));
+      cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
+      const cx<double> dw = cadd(cscale(csub(w, wm_prev), a.dtabm), csub(rw, rw_prev));
+      rhs = cadd(rhs, cscale(dw, a.dzm));
+      wm_prev = w;
+      rw_prev = rw;
+      const double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
+      cx<double> r = cscale(rhs, dz2);
+      double d;
+      if (z == 1) {
+        d = 1.0 / diag;
+      } else {
+        d = 1.0 / (diag - d_prev);               // a(i-1) = b(i) = 1
+        r = csub(r, cscale(r_prev, d_prev));
+      }
+      sr[o] = r;
+      d_prev = d;
+      r_prev = r;
     }
   }
   // backward sweep (tridv.f90:44-53) fused with comp2 (comp2.f90:21-41)
-  double p_next = 0.0;
+  cx<double> p_next = {0, 0};
   for (int zt = nz; zt >= 1; zt -= ZU) {
-    double ld[ZU], lr[ZU];
-    T lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU];
-    if (active) {
+    double ld[ZU];
+    cx<double> lr[ZU];
+    cx<T> lu[ZU], lv[ZU], lw[ZU], lru[ZU], lrv[ZU], lrw[ZU];
 #pragma unroll
-      for (int q = 0; q < ZU; ++q) {
-        const int z = zt - q;
-        if (z >= 1) {
-          const size_t o = (size_t)z * pl + m;
-          ld[q] = a.piv[o];
-          lr[q] = sr[(size_t)z * 2 * pl + lane_id];
-          lu[q] = reinterpret_cast<const T*>(U0 + o)[im];
-          lv[q] = reinterpret_cast<const T*>(U1 + o)[im];
-          lw[q] = reinterpret_cast<const T*>(U2 + o)[im];
-          lru[q] = reinterpret_cast<const T*>(N0 + o)[im];
-          lrv[q] = reinterpret_cast<const T*>(N1 + o)[im];
-          lrw[q] = reinterpret_cast<const T*>(N2 + o)[im];
-        }
+    for (int q = 0; q < ZU; ++q) {
+      const int z = zt - q;
+      if (z >= 1) {
+        const size_t o = (size_t)z * pl + m;
+        ld[q] = a.piv[o]; lr[q] = sr[o];
+        lu[q] = U0[o]; lv[q] = U1[o]; lw[q] = U2[o];
+        lru[q] = N0[o]; lrv[q] = N1[o]; lrw[q] = N2[o];
       }
     }
 #pragma unroll
     for (int q = 0; q < ZU; ++q) {
       const int z = zt - q;
       if (z < 1) break;
-      double p = 0.0;
-      if (active) p = (z == nz) ? lr[q] * ld[q] : (lr[q] - p_next) * ld[q];
-      const double pp = __shfl_xor(p, 1, 64);    // the partner part of p
-      if (active) {
-        const size_t o = (size_t)z * pl + m;
-        // -(i k p) in this lane's part: re: +k p.im, im: -k p.re
-        const double ru = (double)lru[q] - sgn * (pp * kxp);   // comp2.f90:21
-        const double rv = (double)lrv[q] - sgn * (pp * kyp);   // comp2.f90:25
-        double rw = 0.0, wn = 0.0;
-        if (z != nz) {
-          rw = (double)lrw[q] - (p_next - p) * a.dzm;          // comp2.f90:29
-          wn = (double)lw[q] + rw * a.dtab1;                   // comp2.f90:37
-        }
-        store_part(U0 + o, im, (double)lu[q] + ru * a.dtab1);  // comp2.f90:35-36
-        store_part(U1 + o, im, (double)lv[q] + rv * a.dtab1);
-        store_part(U2 + o, im, wn);
-        store_part(R0 + o, im, ru);
-        store_part(R1 + o, im, rv);
-        store_part(R2 + o, im, rw);
-        reinterpret_cast<double*>(a.P + o)[im] = p;
+      const size_t o = (size_t)z * pl + m;
+      const cx<double> p = (z == nz) ? cscale(lr[q], ld[q]) : cscale(csub(lr[q], p_next), ld[q]);
+      const cx<double> ru = csub(widen(lru[q]), cmuli(cscale(p, kxp)));   // comp2.f90:21
+      const cx<double> rv = csub(widen(lrv[q]), cmuli(cscale(p, kyp)));   // comp2.f90:25
+      cx<double> rw = {0, 0}, wn = {0, 0};
+      if (z != nz) {
+        rw = csub(widen(lrw[q]), cscale(csub(p_next, p), a.dzm));        // comp2.f90:29
+        wn = cadd(widen(lw[q]), cscale(rw, a.dtab1));                     // comp2.f90:37
       }
+      st2(U0 + o, cadd(widen(lu[q]), cscale(ru, a.dtab1)));              // comp2.f90:35-36
+      st2(U1 + o, cadd(widen(lv[q]), cscale(rv, a.dtab1)));
+      st2(U2 + o, wn);
+      st2(R0 + o, ru);
+      st2(R1 + o, rv);
+      st2(R2 + o, rw);
+      a.P[o] = p;
       p_next = p;
     }
   }

@@ -72,7 +72,7 @@ struct cf_ctx {
   cx<double>* xs64 = nullptr;
   cx<double>* tw64NX = nullptr;
   double* piv = nullptr;        // k_press reciprocal pivots [slots][modes] (set at create)
-  double* sr = nullptr;         // k_press forward-eliminated rhs [slots][2 modes]
+  cx<double>* sr = nullptr;     // k_press forward-eliminated rhs [slots][modes]
   double* M0 = nullptr;         // FP64 (0,0) coefficients of u, v, ru, rv [4][slots]
   double* part = nullptr;
   double* red = nullptr;        // [7][slots]: divz and the 6 SGS means
@@ -194,7 +194,7 @@ struct EngineT : Engine {
     if ((rc = dalloc(ctx, &ctx->xs64, band)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->tw64NX, (size_t)NX)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->piv, S * NY * MX)) != CF_OK) return rc;
-    if ((rc = dalloc(ctx, &ctx->sr, 2 * S * NY * MX)) != CF_OK) return rc;
+    if ((rc = dalloc(ctx, &ctx->sr, S * NY * MX)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->M0, 4 * S)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->part, 7 * S * NYB)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->red, 7 * S)) != CF_OK) return rc;
@@ -462,8 +462,8 @@ struct EngineT : Engine {
       a.dtab2 = c.dtab2;
       a.dtabm = c.dtabm;
       const int modes = MX * NY;
-      // one lane per real and imaginary part, and one workgroup for mode (0,0)
-      hipLaunchKernelGGL(cf::k_press<T>, dim3((2 * modes + 63) / 64 + 1), dim3(64), 0, s, a);
+      // one lane per mode, and one more workgroup for mode (0,0)
+      hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 63) / 64 + 1), dim3(64), 0, s, a);
     }
     mark(true);
     {
