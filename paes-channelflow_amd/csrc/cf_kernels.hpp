@@ -2499,9 +2499,9 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
       }
       e0 = e0 + (T)a.dtab1 * r.x;
       e1 = e1 + (T)a.dtab1 * r.y;
+      if (!isfinite(e0) || !isfinite(e1)) bad = 1;   // before the clamp, which would turn a NaN into emin
       e0 = e0 > (T)a.emin ? e0 : (T)a.emin;
       e1 = e1 > (T)a.emin ? e1 : (T)a.emin;
-      if (!isfinite(e0) || !isfinite(e1)) bad = 1;
       *pe0 = e0; *pe1 = e1;
       *pr0 = r.x; *pr1 = r.y;
       ea[p] = e0; eb[p] = e1;

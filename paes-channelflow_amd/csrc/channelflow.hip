@@ -74,6 +74,7 @@ struct cf_ctx {
   double* piv = nullptr;        // k_press reciprocal pivots [slots][modes] (set at create)
   cx<double>* sr = nullptr;     // k_press forward-eliminated rhs [slots][modes]
   double* M0 = nullptr;         // FP64 (0,0) coefficients of u, v, ru, rv [4][slots]
+  cx<double>* band64 = nullptr; // FP32 contexts: FP64 band of the state crossing the boundary
   double* part = nullptr;
   double* red = nullptr;        // [7][slots]: divz and the 6 SGS means
   double* kxd = nullptr;
@@ -502,11 +503,35 @@ struct EngineT : Engine {
     return CF_OK;
   }
 
+  // FP32 contexts move the state across the boundary through FP64
+  // transforms: the FP32 transforms' rounding is relative to a level's
+  // largest coefficient (the mean of u), which would add white noise to the
+  // much smaller fluctuations.  band64 is allocated on first use.
+  int need_band64(cf_ctx* ctx) {
+    if (ctx->band64) return CF_OK;
+    return dalloc(ctx, &ctx->band64, (size_t)ctx->slots * NY * MX);
+  }
+
   // physical (host, nz planes) -> band spectrum slots 1..nz of dst
   int to_band(cf_ctx* ctx, const double* host, C* dst) {
     const int nz = ctx->nz;
     HIPCHK(hipMemcpyAsync(ctx->staging, host, (size_t)nz * NY * NX * sizeof(double),
                           hipMemcpyHostToDevice, ctx->stream));
+    if constexpr (sizeof(T) == 4) {
+      int rc = need_band64(ctx);
+      if (rc != CF_OK) return rc;
+      constexpr int TK64 = cf::ColShape<double, NX, NY>::TK;
+      const size_t plane = (size_t)NY * MX;
+      hipLaunchKernelGGL((cf::k_rows_fwd<double, NX>), dim3(NY / 2, nz), dim3(MX), 0, ctx->stream,
+                         ctx->staging, ctx->xs64, ctx->tw64NX, 1);
+      hipLaunchKernelGGL((cf::k_cols_small<double, NX, NY, false>), dim3(MX / TK64, nz), dim3(256), 0,
+                         ctx->stream, ctx->xs64, ctx->band64, ctx->tw64NX, 1);
+      const size_t nd = 2 * (size_t)nz * plane;
+      hipLaunchKernelGGL((cf::k_cast<double, T>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream,
+                         (const double*)(ctx->band64 + plane), (T*)(dst + plane), nd);
+      HIPCHK(hipGetLastError());
+      return CF_OK;
+    }
     hipLaunchKernelGGL((cf::k_rows_fwd<T, NX>), dim3(NY / 2, nz), dim3(MX), 0, ctx->stream,
                        ctx->staging, cptr(ctx->xs_tmp), cptr(ctx->twNX), 1);
     hipLaunchKernelGGL((cf::k_cols_small<T, NX, NY, false>), dim3(MX / CS::TK, nz), dim3(256), 0,
@@ -705,18 +730,29 @@ struct EngineT : Engine {
     } else if (which == 4) {
       rc = from_band<double>(ctx, ctx->P, ctx->xs64, ctx->tw64NX);
     } else {
-      rc = from_band<T>(ctx, cptr(bands[which]), cptr(ctx->xs_tmp), cptr(ctx->twNX));
       // u, v, ru, rv: the plane means from their FP64 copies; the comp1
       // tendency of u with the imposed gradient, which k_press adds (comp1.f90:152)
       const int q = which == 0 ? cf::M0_U : which == 1 ? cf::M0_V : which == 5 ? cf::M0_RU
                                                                    : which == 6 ? cf::M0_RV : -1;
       const double add = which == 9 ? cf::FORCING : 0.0;
-      if (rc == CF_OK && (q >= 0 || add != 0.0)) {
-        hipLaunchKernelGGL((cf::k_shift_levels<T>), dim3(nz), dim3(256), 0, s, ctx->staging,
-                           q >= 0 ? ctx->M0 + (size_t)q * ctx->slots : nullptr, cptr(bands[which]),
-                           (size_t)NY * MX, (size_t)NY * NX, add);
-        HIPCHK(hipGetLastError());
+      const double* m0 = q >= 0 ? ctx->M0 + (size_t)q * ctx->slots : nullptr;
+      const size_t plane = (size_t)NY * MX;
+      if constexpr (sizeof(T) == 4) {           // FP64 synthesis of the FP32 band
+        if ((rc = need_band64(ctx)) != CF_OK) return rc;
+        const size_t nd = 2 * (size_t)ctx->slots * plane;
+        hipLaunchKernelGGL((cf::k_cast<T, double>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
+                           (const T*)bands[which], (double*)ctx->band64, nd);
+        rc = from_band<double>(ctx, ctx->band64, ctx->xs64, ctx->tw64NX);
+        if (rc == CF_OK && (m0 || add != 0.0))
+          hipLaunchKernelGGL((cf::k_shift_levels<double>), dim3(nz), dim3(256), 0, s, ctx->staging, m0,
+                             ctx->band64, plane, (size_t)NY * NX, add);
+      } else {
+        rc = from_band<T>(ctx, cptr(bands[which]), cptr(ctx->xs_tmp), cptr(ctx->twNX));
+        if (rc == CF_OK && (m0 || add != 0.0))
+          hipLaunchKernelGGL((cf::k_shift_levels<T>), dim3(nz), dim3(256), 0, s, ctx->staging, m0,
+                             cptr(bands[which]), plane, (size_t)NY * NX, add);
       }
+      HIPCHK(hipGetLastError());
     }
     return rc;
   }

@@ -3,11 +3,10 @@
 
 * The FP32 path (FP32 fields, transforms, products and SGS; FP64 pressure,
   projection and update) after 200 steps, against the FP64 run from the same
-  state.  The error is printed per output and asserted within:
-  - u, v, w, e, p: 1e-3 relative L2;
-  - the mean profile uxym: 1e-4 relative;
-  - the resolved TKE englez: 1e-3 relative;
-  - the calc_spectra x-y spectra: 1e-3 relative at every wavenumber.
+  state.  The error is printed per output and asserted at about 10x the
+  measured value (u 2.0e-7, v 4.4e-5, w 3.5e-5, e 1.1e-5, p 8.8e-4 relative
+  L2; mean profile 1.2e-8, resolved TKE 3.3e-6, calc_spectra x-y spectra
+  2.4e-5 at every wavenumber).
 * The FP32 + learned-closure path after 200 steps, against the FP64
   TKE-closure run.  Asserted: the mean profile within 1 % (measured 0.24 %)
   and the plane- and level-summed resolved TKE within 5 % (measured 1.7 %).
@@ -19,8 +18,12 @@
   near the walls, and the fluctuation fields decorrelate within 200 steps.
 * The same with the network on the nnx x nny grid (CF_CLOSURE_GRID_SMALL).
 * FP32 against FP64 after 2000 steps at 256^2 x 128 (the metric
-  configuration's comparison): mean profile and resolved TKE within 1e-3,
-  x-y spectra within 1e-2 up to kmax/4 and 2e-2 over all k.
+  configuration's comparison), per output: fields u 2e-5, e 5e-4, v, w, p
+  5e-3; mean profile 2e-5, resolved TKE 3e-4, x-y spectra 3e-3 up to kmax/4
+  and 5e-3 over all k (each about 10x the measured error).
+* The FP32 path's mean mode: on a horizontally uniform flow the FP32 mean
+  profile follows the FP64 path's within 2e-7 over 1000 steps, because the
+  plane means of u, v, ru, rv and the imposed gradient are kept in FP64.
 * 32^3 for 100 steps (the CPU-reference case): FP64 fields within 1e-9 of
   the oracle.
 """
@@ -94,9 +97,10 @@ def test_fp32_path_error_per_output_256():
     s64, m64 = _run(256, 128, 200, C.CF_PREC_FP64)
     s32, m32 = _run(256, 128, 200, C.CF_PREC_FP32)
     e = _report("fp32 vs fp64 after 200 steps:", s32, s64, m32, m64, 128)
+    bounds = dict(u=2e-6, v=5e-4, w=5e-4, e=1e-4, p=5e-3)   # about 10x the measured errors
     for k in "uvwep":
-        assert e[k] < 1e-3, (k, e)
-    assert e["uxym"] < 1e-4 and e["englez"] < 1e-3 and e["spectra"] < 1e-3, e
+        assert e[k] < bounds[k], (k, e)
+    assert e["uxym"] < 1e-6 and e["englez"] < 5e-5 and e["spectra"] < 3e-4, e
 
 
 def test_learned_path_against_fp64_tke_256():
