@@ -33,7 +33,7 @@ extern "C" {
 #define CF_E_ARG (-1)         /* invalid argument or unsupported option          */
 #define CF_E_HIP (-2)         /* HIP runtime error (allocation, launch, copy)     */
 #define CF_E_UTAU (-3)        /* utau > 10 wall-model guard: lowerbc.f90:77-80,    */
-                              /* upperbc.f90:253-256 (the reference calls lesabort) */
+                              /* upperbc.f90:76-79 (the reference calls lesabort) */
 #define CF_E_NONFINITE (-4)   /* a field became NaN/Inf                           */
 #define CF_E_ECLAMP (-5)      /* reserved: no longer returned (the comp1 clamp of  */
                               /* the expanded e, comp1.f90:109,121, is carried out) */
@@ -83,17 +83,17 @@ int cf_set_state(cf_ctx* ctx, const double* u, const double* v, const double* w,
 /* Seeded initial condition, written to the device as by cf_set_state.
  * u = u_tau/0.4 ln(z_w/z0) + amp u_tau xi, v = amp u_tau xi, w = amp u_tau xi
  * (faces 1..nnz-1; w(nnz) = 0), e = 0.01 u_tau^2, with u_tau = sqrt(0.016 zl/2),
- * z_w = min(zm, zl-zm), zm = (iz-1/2) dz (the log profile of init.f90:484-491)
+ * z_w = min(zm, zl-zm), zm = (iz-1/2) dz (the log profile of init.f90:71-78)
  * and xi in [-1,1) from SplitMix64 keyed by (seed, field, i, j, iz).  Replaces
- * initvel (init.f90:463-527), whose random stream is compiler-specific. */
+ * initvel (init.f90:50-114), whose random stream is compiler-specific. */
 int cf_seeded_init(cf_ctx* ctx, uint64_t seed, double amp);
 
 /* Read the interior levels 1..nnz of u, v, w, e and p (any pointer may be
- * NULL).  Replaces the per-level record of sav_field (read_write.f90:117-126). */
+ * NULL).  Replaces the per-level record of sav_field (read_write.f90:31-40). */
 int cf_get_state(cf_ctx* ctx, double* u, double* v, double* w, double* e, double* p);
 
 /* Read the RK registers ru, rv, rw, re (levels 1..nnz, pressure gradient
- * included: the values comp2 leaves behind, comp2.f90:238-254).  Any pointer
+ * included: the values comp2 leaves behind, comp2.f90:17-31).  Any pointer
  * may be NULL. */
 int cf_get_tendency(cf_ctx* ctx, double* ru, double* rv, double* rw, double* re);
 
@@ -103,7 +103,7 @@ int cf_get_comp1_tendency(cf_ctx* ctx, double* ru, double* rv, double* rw);
 
 /* One RK3 stage istage in {1,2,3} with the fixed dt: the body of the stage
  * loop les.F90:254-285 -- lower, upper, comp1, comprhs, press, comp2.  When
- * mnout != 0 and istage == 1, the SGS stress means of compsgs3.f90:236-243 are
+ * mnout != 0 and istage == 1, the SGS stress means of compsgs3.f90:71-78 are
  * accumulated.  Enqueued on the context stream. */
 int cf_stage(cf_ctx* ctx, int istage, int mnout);
 
@@ -121,11 +121,11 @@ int cf_sync(cf_ctx* ctx);
 int cf_get_divz(cf_ctx* ctx, double* divz);
 
 /* utau, utau_s, dudt, dvdt as left by sufsim/sufsim_upper (lowerbc.f90:67-72,
- * upperbc.f90:251): 4 doubles. */
+ * upperbc.f90:74): 4 doubles. */
 int cf_get_scalars(cf_ctx* ctx, double* scalars);
 
 /* Mean profiles (compmn.F90:1-64) into meanbuf[19*nnz] in the row order of
- * allocate.f90:272-294 (englez, engsbz, divz, uxym, vxym, wxym, uule, vvle,
+ * allocate.f90:89-111 (englez, engsbz, divz, uxym, vxym, wxym, uule, vvle,
  * wwle, uwle, vwle, uvle, pple, uusb, vvsb, uvsb, wwsb, uwsb, vwsb), stored
  * meanbuf[row + 19*(iz-1)] like the Fortran meanbuf(19,nnz).  Computed on
  * the device from the band spectra (Parseval); only the profiles cross PCIe. */
@@ -170,7 +170,7 @@ const char* cf_last_error(const cf_ctx* ctx);
  * resolved velocity gradients at a cell centre (ux uy uz vx vy vz wx wy wz),
  * the filter width dsl (les.F90:118) and the wall distance min(z, zl-z) to
  * the SGS stresses (t11 t22 t33 t12 t13 t23).  It replaces the stresses of
- * compsgs1 (compsgs1.f90:104-156) at the same staggered locations, with the
+ * compsgs1 (compsgs1.f90:1-53) at the same staggered locations, with the
  * TKE closure as fallback wherever the prediction is not realizable or
  * backscatters.  BF16 weights and activations on MFMA with FP32
  * accumulation; FP32 master weights and Adam (beta 0.9/0.999, eps 1e-8).
@@ -256,7 +256,7 @@ int cf_get_closure_counts(cf_ctx* ctx, int64_t* counts);
  * - e (cell centre);
  * - uu, vv, ww, uv, uw, vw of the cell-centred fine velocities;
  * - the fine grid's own TKE-closure stresses t11 t22 t33 t12 t13 t23 at cell
- *   centres (compsgs1.f90:144-153; face stresses averaged to the centre). */
+ *   centres (compsgs1.f90:41-50; face stresses averaged to the centre). */
 int cf_filter_snapshot(cf_ctx* ctx, int32_t fx, int32_t fy, int32_t fz, double* out);
 
 #ifdef __cplusplus

@@ -9,11 +9,11 @@ is missing.
 It restates, in vectorised numpy (one array axis per grid direction, all
 levels at once), the algorithm of the reference's Fortran stage:
 
-    lower (lowerbc.f90:1-20) -> upper (upperbc.f90:178-207)
+    lower (lowerbc.f90:1-20) -> upper (upperbc.f90:1-30)
     -> comp1 (comp1.f90:1-188: derivsbot/derivs, compadv1, complen, compsgs1,
        sufsim/sufsim_upper, compsgs3, compress)
-    -> comprhs (comprhs.f90:1-37) -> press (press.F90:45-159, tridv.f90)
-    -> comp2 (comp2.f90:224-268)
+    -> comprhs (comprhs.f90:1-37) -> press (press.F90:1-115, tridv.f90)
+    -> comp2 (comp2.f90:1-45)
 
 with the spectral layer of filter_fftw.f90 (ffthrz, xderiv, yderiv, expand,
 compress, deal) written on numpy's pocketfft.  The reference links FFTW 2.x
@@ -61,7 +61,7 @@ def lev(a, L):
 
 
 # ----------------------------------------------------------------------------
-# Parameters and derived constants (les.F90:92-121, init.f90:417-457)
+# Parameters and derived constants (les.F90:92-121, init.f90:4-44)
 # ----------------------------------------------------------------------------
 @dataclass
 class Params:
@@ -73,7 +73,7 @@ class Params:
     zl: float = 2.0
     z0: float = 7.519e-4
     dt: float = 0.01
-    imodel: int = 3           # 3 = Moeng84 TKE length, 4 = Smagorinsky (complen.f90:87-92)
+    imodel: int = 3           # 3 = Moeng84 TKE length, 4 = Smagorinsky (complen.f90:19-24)
     isurface: int = 1         # wall model 1..6 (lowerbc.f90:126-161); 1 = Schumann
     ufree: float = 0.0        # the uninitialised ufree of lowerbc.f90:38,63 (see wall_stress)
     alk: float | None = None  # SGS length scale (allocate.f90:61, never set; None -> dsl)
@@ -103,7 +103,7 @@ class Params:
         if self.alk is None:
             self.alk = self.dsl
         self.xkmax = self.dz * self.dz / self.dt / 5.0   # les.F90:216
-        # wavenumbers (init.f90:428-444)
+        # wavenumbers (init.f90:15-31)
         self.xk = _waves(nnx, self.xl, nnx // 2 + 1)
         self.yk = _waves(nny, self.yl, nny // 2 + 1)
         self.xkb = _waves(self.nnxb, self.xl, self.nnxb // 2 + 1)
@@ -111,7 +111,7 @@ class Params:
 
 
 def _waves(n, length, ncut):
-    """xk(ix) = (ix-1) 2pi/L, negative above ncut (init.f90:428-444)."""
+    """xk(ix) = (ix-1) 2pi/L, negative above ncut (init.f90:15-31)."""
     k = np.arange(n, dtype=np.float64) * 2.0 * math.pi / length
     ix = np.arange(1, n + 1)
     k[ix > ncut] = -(n - ix[ix > ncut] + 1) * 2.0 * math.pi / length
@@ -269,7 +269,7 @@ def initial_fields(pr: Params, seed: int = 10972, amp: float = 0.1):
     u = u_tau/kappa ln(z_w/z0) + amp u_tau xi, v = amp u_tau xi,
     w = amp u_tau xi at faces 1..nnz-1 (w(nnz) = 0, the top wall),
     e = 0.01 u_tau^2, with z_w = min(zm, zl - zm), zm = (iz - 1/2) dz.
-    The log profile follows init.f90:484-491.  Returns arrays (nnz, nny, nnx)
+    The log profile follows init.f90:71-78.  Returns arrays (nnz, nny, nnx)
     *before* the dealiasing that set_state applies (les.F90:185-190)."""
     ut = utau_forcing(pr.zl)
     zm = (np.arange(1, pr.nnz + 1) - 0.5) * pr.dz
@@ -294,7 +294,7 @@ def set_state(pr: Params, u, v, w, e) -> State:
 
 
 # ----------------------------------------------------------------------------
-# Boundary fills (lowerbc.f90:1-20, upperbc.f90:178-207)
+# Boundary fills (lowerbc.f90:1-20, upperbc.f90:1-30)
 # ----------------------------------------------------------------------------
 def lower(st: State, pr: Params):
     """u,v(0)=0; w(-1:0)=0; e(0)=e(1)  (np=(2,2,2,1), nm=(1,1,2,1))."""
@@ -438,7 +438,7 @@ def sgs_fields(st: State, pr: Params):
     wb[0 + OF: n + 2 + OF] = expand(st.w[0 + OF: n + 2 + OF], nxb, nyb)      # faces 0..nnz+1
     eb[0 + OF: n + 2 + OF] = np.maximum(expand(st.e[0 + OF: n + 2 + OF], nxb, nyb), EMIN)
 
-    # --- gradients (derivs1.f90:200-265, derivs2.f90:271-312)
+    # --- gradients (derivs1.f90:1-66, derivs2.f90:1-42)
     ux = np.zeros_like(ub); uy = np.zeros_like(ub); vx = np.zeros_like(ub); vy = np.zeros_like(ub)
     wz = np.zeros_like(ub); uz = np.zeros_like(ub); vz = np.zeros_like(ub)
     wx = np.zeros_like(ub); wy = np.zeros_like(ub); ex = np.zeros_like(ub); ey = np.zeros_like(ub)
@@ -449,14 +449,14 @@ def sgs_fields(st: State, pr: Params):
     fs = slice(1 + OF, n + OF)                                      # faces 1..nnz-1
     uz[fs] = (ub[2 + OF: n + 1 + OF] - ub[fs]) * pr.dzm
     vz[fs] = (vb[2 + OF: n + 1 + OF] - vb[fs]) * pr.dzm
-    uz[0 + OF] = uz[1 + OF]; vz[0 + OF] = vz[1 + OF]                # derivs1.f90:236-237
-    uz[n + OF] = uz[n - 1 + OF]; vz[n + OF] = vz[n - 1 + OF]        # derivs2.f90:297-298
+    uz[0 + OF] = uz[1 + OF]; vz[0 + OF] = vz[1 + OF]                # derivs1.f90:37-38
+    uz[n + OF] = uz[n - 1 + OF]; vz[n + OF] = vz[n - 1 + OF]        # derivs2.f90:27-28
     wx[fs] = xderiv(wb[fs], xl); wy[fs] = yderiv(wb[fs], yl)        # faces 0, nnz stay 0
     es = slice(1 + OF, n - 1 + OF)                                  # faces 1..nnz-2
     ex[es] = xderiv(eb[es], xl); ey[es] = yderiv(eb[es], yl)
-    ex[n - 1 + OF] = ex[n - 2 + OF]; ey[n - 1 + OF] = ey[n - 2 + OF]  # stale at nnz-1 (derivs2.f90:279)
+    ex[n - 1 + OF] = ex[n - 2 + OF]; ey[n - 1 + OF] = ey[n - 2 + OF]  # stale at nnz-1 (derivs2.f90:9,20-21)
 
-    # --- eddy diffusivity at faces 0..nnz (complen.f90:69-96)
+    # --- eddy diffusivity at faces 0..nnz (complen.f90:1-28)
     xkk = np.zeros_like(ub)
     fa = slice(0 + OF, n + 1 + OF)
     if pr.imodel == 3:
@@ -468,9 +468,9 @@ def sgs_fields(st: State, pr: Params):
                  + (uz[fa] + wx[fa]) ** 2 + (vz[fa] + wy[fa]) ** 2)
         xkk[fa] = np.minimum((0.1 * pr.dsl) ** 2 * np.sqrt(shear), pr.xkmax)
     else:
-        raise NotImplementedError("imodel must be 3 or 4 (complen.f90:87-92)")
+        raise NotImplementedError("imodel must be 3 or 4 (complen.f90:19-24)")
 
-    # --- stresses (compsgs1.f90:104-156)
+    # --- stresses (compsgs1.f90:1-53)
     km = slice(0 + OF, n + OF)        # face k-1 for levels 1..nnz
     k0 = slice(1 + OF, n + 1 + OF)    # face k
     xs = xkk[km] + xkk[k0]
@@ -530,7 +530,7 @@ def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
            - (wp * ep - wm * em) * pr.todzm)
     rub = rub + FORCING                                             # comp1.f90:152
 
-    # --- SGS terms (compsgs3.f90:166-245)
+    # --- SGS terms (compsgs3.f90:1-80)
     xk0 = xkk[k0]; xkp = xkk[2 + OF: n + 2 + OF]; xkm = xkk[km]
     temp = (xderiv(2.0 * xk0 * ex[sl], xl) + yderiv(2.0 * xk0 * ey[sl], yl)
             + ((xkp + xk0) * (ep - e0) - (xk0 + xkm) * (e0 - em)) * pr.dzm2)
@@ -539,7 +539,7 @@ def comp1(st: State, pr: Params, istage: int, mnout: bool, dtab2: float):
     rub = rub - xderiv(tauuu, xl) - yderiv(tauuv, yl) - (tauuw[k0] - tauuw[km]) * pr.dzm
     rvb = rvb - xderiv(tauuv, xl) - yderiv(tauvv, yl) - (tauvw[k0] - tauvw[km]) * pr.dzm
     rwb = rwb - xderiv(tauuw[k0], xl) - yderiv(tauvw[k0], yl) - (tauww[2 + OF: n + 2 + OF] - tauww[sl]) * pr.dzm
-    rwb[n - 1] = 0.0                                                # level nnz: compsgs3.f90:225-229
+    rwb[n - 1] = 0.0                                                # level nnz: compsgs3.f90:60-64
     if mnout and istage == 1:
         mean = lambda a: np.sum(a, axis=(-2, -1)) * pr.xymb  # noqa: E731
         st.sgs = dict(uusb=mean(tauuu), vvsb=mean(tauvv), uvsb=mean(tauuv),
@@ -589,7 +589,7 @@ def comprhs(st: State, pr: Params, dtabm: float):
 
 
 def tridv(sub, diag, sup, r):
-    """Thomas algorithm with reciprocal pivots (tridv.f90:167-221).
+    """Thomas algorithm with reciprocal pivots (tridv.f90:1-55).
 
     Arrays have the z index first: (n, ...).  b(1) and a(n) are not used."""
     d = diag.astype(np.float64).copy()
@@ -609,13 +609,13 @@ def tridv(sub, diag, sup, r):
 
 
 def k2_table(pr: Params):
-    """xk2 over the half spectrum (init.f90:446-455): xk(ix)^2 + yk(iy)^2."""
+    """xk2 over the half spectrum (init.f90:33-42): xk(ix)^2 + yk(iy)^2."""
     ncx = pr.nnx // 2 + 1
     return pr.xk[:ncx][None, :] ** 2 + pr.yk[:, None] ** 2       # (nny, ncx)
 
 
 def press(rhs, pr: Params):
-    """press.F90:45-159 (serial): FFT, tridiagonal z-solve per (kx,ky) with
+    """press.F90:1-115 (serial): FFT, tridiagonal z-solve per (kx,ky) with
     Neumann rows at 1 and nnz, (0,0) mode pinned to 0, Nyquist rows zeroed."""
     n = pr.nnz
     P = ffthrz_fwd(rhs)                                # (n, ny, ncx)
@@ -627,18 +627,18 @@ def press(rhs, pr: Params):
     sub = np.ones(P.shape)
     sup = np.ones(P.shape)
     r = dzdz * P
-    # kx=ky=0 mode: identity with zero right-hand side (press.F90:125-132)
+    # kx=ky=0 mode: identity with zero right-hand side (press.F90:81-88)
     sub[:, 0, 0] = 0.0; diag[:, 0, 0] = 1.0; sup[:, 0, 0] = 0.0; r[:, 0, 0] = 0.0
     Ph = tridv(sub, diag, sup, r)
-    Ph[:, 0, 0] = 0.0                                  # press.F90:151-152
-    Ph[:, :, pr.nnx // 2] = 0.0                        # press.F90:153
+    Ph[:, 0, 0] = 0.0                                  # press.F90:107-108
+    Ph[:, :, pr.nnx // 2] = 0.0                        # press.F90:109
     if pr.nny != 1:
-        Ph[:, pr.nny // 2, :] = 0.0                    # press.F90:154
+        Ph[:, pr.nny // 2, :] = 0.0                    # press.F90:110
     return ffthrz_inv(Ph, pr.nnx), Ph
 
 
 def comp2(st: State, pr: Params, dtab1: float):
-    """Pressure gradient, RK update, clamp and mean removal (comp2.f90:224-268)."""
+    """Pressure gradient, RK update, clamp and mean removal (comp2.f90:1-45)."""
     n = pr.nnz
     sl = slice(1 + OF, n + 1 + OF)
     ns = slice(1 + OF, n + OF)
@@ -686,12 +686,12 @@ def step(st: State, pr: Params, mnout: bool = False):
 # ----------------------------------------------------------------------------
 MEAN_NAMES = ("englez", "engsbz", "divz", "uxym", "vxym", "wxym", "uule", "vvle", "wwle",
               "uwle", "vwle", "uvle", "pple", "uusb", "vvsb", "uvsb", "wwsb", "uwsb", "vwsb")
-# meanbuf row order: allocate.f90:272-294
+# meanbuf row order: allocate.f90:89-111
 
 
 def compmn(st: State, pr: Params):
     """Horizontal-mean profiles and resolved moments (compmn.F90:1-64).
-    Returns meanbuf as (19, nnz) in the allocate.f90:272-294 row order."""
+    Returns meanbuf as (19, nnz) in the allocate.f90:89-111 row order."""
     n = pr.nnz
     avg = lambda a: np.sum(a, axis=(-2, -1)) * pr.xym  # noqa: E731
     u = st.u; v = st.v; w = st.w

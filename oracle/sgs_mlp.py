@@ -25,7 +25,7 @@ What it restates:
   - wz = (w(k) - w(k-1)) / dz;
   - the filter width dsl (les.F90:118);
   - the wall distance min(zm, zl - zm);
-* the insertion at the stress computation of compsgs1.f90:104-156.  The
+* the insertion at the stress computation of compsgs1.f90:1-53.  The
   stresses sit at the same staggered locations, with the TKE closure as
   fallback:
   - a cell uses the network's deviatoric stress when the predicted stress is
@@ -34,7 +34,7 @@ What it restates:
   - a face (tau13, tau23) uses the mean of its two cells' network values when
     both cells passed, and the TKE face stress otherwise;
   - the wall faces keep the wall model (lowerbc.f90:129-130,
-    upperbc.f90:304-305).
+    upperbc.f90:127-128).
 
 Pinning: the backward pass against central finite differences of the FP64
 forward; Adam against its closed form after one step (m/(sqrt(v)+eps) =
@@ -256,7 +256,7 @@ class LearnedClosure:
 
 def tke_stress_apriori(feats, e_sgs, alk, coef=0.1):
     """Deviatoric TKE-closure stress -2 K S_ij with K = coef*alk*sqrt(e)
-    (complen.f90:88-89, compsgs1.f90:144-153), in STRESS_NAMES order."""
+    (complen.f90:20-21, compsgs1.f90:41-50), in STRESS_NAMES order."""
     ux, uy, uz, vx, vy, vz, wx, wy, wz = (feats[..., i] for i in range(9))
     K = coef * alk * np.sqrt(np.maximum(e_sgs, 0.0))
     return np.stack([-2 * K * ux, -2 * K * vy, -2 * K * wz, -K * (uy + vx), -K * (uz + wx),
@@ -286,8 +286,8 @@ def small_grid_stresses(st, pr: O.Params, net):
     v = st.v[0 + OF: n + 2 + OF]
     w = st.w[0 + OF: n + 1 + OF]                     # faces 0..nnz
     e = st.e[0 + OF: n + 1 + OF].copy()              # faces 0..nnz
-    e[0] = e[1]                                      # lower (lowerbc.f90:13)
-    e[n] = e[n - 1]                                  # upper (upperbc.f90:197-201)
+    e[0] = e[1]                                      # lower (lowerbc.f90:15)
+    e[n] = e[n - 1]                                  # upper (upperbc.f90:20-24)
     uk, vk = u[1:n + 1], v[1:n + 1]
     ux, uy = O.xderiv(uk, xl), O.yderiv(uk, yl)
     vx, vy = O.xderiv(vk, xl), O.yderiv(vk, yl)
@@ -299,12 +299,12 @@ def small_grid_stresses(st, pr: O.Params, net):
     zm = (np.arange(1, n + 1) - 0.5) * pr.dz
     wall = np.minimum(zm, pr.zl - zm)[:, None, None] * np.ones_like(uz)
     feats = np.stack([ux, uy, uz, vx, vy, vz, wx, wy, wz, np.full_like(uz, pr.dsl), wall], axis=-1)
-    K = np.minimum(0.1 * pr.alk * np.sqrt(e), pr.xkmax)          # complen.f90:88-89, faces 0..nnz
+    K = np.minimum(0.1 * pr.alk * np.sqrt(e), pr.xkmax)          # complen.f90:20-21, faces 0..nnz
     xs = K[:-1] + K[1:]
-    tke = (-xs * ux, -xs * vy, -xs * 0.5 * (uy + vx), -xs * wz)   # compsgs1.f90:144-151
+    tke = (-xs * ux, -xs * vy, -xs * 0.5 * (uy + vx), -xs * wz)   # compsgs1.f90:41-48
     uzf = (u[2:n + 1] - u[1:n]) * pr.dzm                          # faces 1..nnz-1
     vzf = (v[2:n + 1] - v[1:n]) * pr.dzm
-    tke13 = -K[1:n] * (uzf + wxf[1:n])                            # compsgs1.f90:152-153
+    tke13 = -K[1:n] * (uzf + wxf[1:n])                            # compsgs1.f90:49-50
     tke23 = -K[1:n] * (vzf + wyf[1:n])
     tau = net.stresses(feats)
     ok = accept(tau, feats)
@@ -372,6 +372,6 @@ class LearnedClosureSmall:
         st.ru[1 + OF: n + 1 + OF] += ru
         st.rv[1 + OF: n + 1 + OF] += rv
         st.rw[1 + OF: n + OF] += rw
-        if means:                                  # the SGS means of compsgs3.f90:236-243
+        if means:                                  # the SGS means of compsgs3.f90:71-78
             for k, v in self._means.items():
                 st.sgs[k] = st.sgs[k] + v

@@ -6,7 +6,7 @@ mirrors the reference driver's vocabulary: a run is created from the
 GRIDSIZE/PARAMS/TIMING namelist values (les.F90:30-34), fields are loaded like
 a restart file (read_field, read_write.f90:1-11), advanced stage by stage
 (lower/upper/comp1/comprhs/press/comp2, les.F90:254-285) and read back like
-sav_field (read_write.f90:117-126).
+sav_field (read_write.f90:31-40).
 
 The shared library must have been built (``__graft_entry__.build()``); if it
 is missing, importing this package raises -- there is no CPU fallback.
@@ -257,7 +257,7 @@ class ChannelFlow:
         return dict(zip(("utau", "utau_s", "dudt", "dvdt"), out))
 
     def compmn(self):
-        """meanbuf(19, nnz) in the allocate.f90:272-294 row order."""
+        """meanbuf(19, nnz) in the allocate.f90:89-111 row order."""
         out = np.empty((self.params.nnz, 19))
         self._check(self.lib.cf_compmn(self._h, _ptr(out)))
         return out.T.copy()

@@ -21,7 +21,7 @@ The samples come from this code's own finer-grid run:
 
 ``evaluate`` measures a trained network a priori on held-out samples.  It
 compares the deviatoric stress error against three baselines:
-- the TKE closure's -2 K S_ij with K = 0.1 dsl sqrt(e) (complen.f90:88-89);
+- the TKE closure's -2 K S_ij with K = 0.1 dsl sqrt(e) (complen.f90:20-21);
 - that closure with its best least-squares coefficient;
 - no model.
 Here e is the resolved subfilter energy plus the filtered fine-grid e.
@@ -110,7 +110,7 @@ def deviatoric(t):
 
 
 def tke_stress(feats, e_sgs, coef=0.1):
-    """-2 K S_ij, K = coef * delta * sqrt(e) (complen.f90:88-89 with alk = dsl)."""
+    """-2 K S_ij, K = coef * delta * sqrt(e) (complen.f90:20-21 with alk = dsl)."""
     f = feats.astype(np.float64)
     K = coef * f[:, 9] * np.sqrt(np.maximum(e_sgs.astype(np.float64), 0.0))
     ux, uy, uz, vx, vy, vz, wx, wy, wz = (f[:, i] for i in range(9))

@@ -11,7 +11,7 @@
 //
 // Stage pipeline (the reference's comp1 -> comprhs -> press -> comp2):
 //   k_wall       plane means of level 1 / nnz -> utau, wall-stress factors
-//                (sufsim, lowerbc.f90:57-91; sufsim_upper, upperbc.f90:241-267)
+//                (sufsim, lowerbc.f90:57-91; sufsim_upper, upperbc.f90:64-90)
 //   k_col_synth  y-inverse of u,v,w,e and d/dy of each onto the 3/2 grid
 //                (expand + yderiv, filter_fftw.f90:194-278)
 //   k_rows       per 3/2-grid row and z-chunk: x-inverse of the 12 fields
@@ -25,7 +25,7 @@
 //                (comp1.f90:126-133), Poisson RHS (comprhs.f90), Thomas
 //                solve (press.F90, tridv.f90) in FP64 and, on the way back
 //                up, the projection and RK update (comp2.f90)
-//   k_erows      the SGS TKE update in physical space (comp2.f90:261) and
+//   k_erows      the SGS TKE update in physical space (comp2.f90:38) and
 //                its x-forward for the next stage
 #pragma once
 #include "cf_fft.hpp"
@@ -719,8 +719,8 @@ __global__ void __launch_bounds__(3 * NX / 2) k_wall_plane(const cx<T>* __restri
 // (derivs2.f90:20-21).  The row kernels differentiate the band-limited e, so
 // for every marked face f the derivatives of the clamp residual
 // d = max(e, emin) - e are added to the e-fluxes of the levels that read
-// ex, ey of face f (level f; also nnz-1 for f = nnz-2, derivs2.f90:279):
-//   F_x^e += 2 K dx(d),  F_y^e += 2 K dy(d)   (compsgs3.f90:185-200),
+// ex, ey of face f (level f; also nnz-1 for f = nnz-2, derivs2.f90:9,20-21):
+//   F_x^e += 2 K dx(d),  F_y^e += 2 K dy(d)   (compsgs3.f90:20-35),
 // with K at the level's face.  dx, dy are xderiv/yderiv on the 3/2 grid
 // (k = 0 and the Nyquist mode dropped, filter_fftw.f90:138-248).
 //   k_eclamp_rows  d and dx(d)                  grid NYB, one thread per point
@@ -851,7 +851,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_eclamp_flux(EclampArgs<T> a) {
     T K;
     if (a.c.imodel == 4) {
       K = a.X[(size_t)iz * pp + (size_t)y * NXB + tid];
-    } else {                                                    // complen.f90:88-89 on the clamped e
+    } else {                                                    // complen.f90:20-21 on the clamped e
       const cx<T>* src = a.B + 6 * a.bstride + (size_t)iz * plane + (size_t)y * MX;
       fill_pair<T, NX>(buf, src, false, nullptr, false, a.kxp, tid, NT);
       __syncthreads();
@@ -1003,7 +1003,7 @@ k_rows(RowArgs<T> a) {
 
   auto xkk_of = [&](T e) {
     T v = T(0.1) * alk * sqrt(e);
-    return v < xkmax ? v : xkmax;   // complen.f90:89
+    return v < xkmax ? v : xkmax;   // complen.f90:21
   };
 
   // window registers (levels iz-1, iz, iz+1 of u, v; faces of w, e)
@@ -1041,7 +1041,7 @@ k_rows(RowArgs<T> a) {
   // the six synthesis pairs of level iz: u,v at iz+1; w,e at face iz+1;
   // (du/dx, du/dy), (dv/dx, dv/dy), (dw/dx, dw/dy) at iz; (de/dx, de/dy) at
   // face iz -- except at iz = nnz-1, where ex, ey are not refreshed
-  // (derivs2.f90:279) and face nnz-2 is reused.  The first member of pairs
+  // (derivs2.f90:9,20-21) and face nnz-2 is reused.  The first member of pairs
   // 2..5 takes an i kx factor.
   auto level_srcs = [&](int iz, const cx<T>** sa, const cx<T>** sb) {
     const int Lp = iz + 1;
@@ -1137,17 +1137,17 @@ k_rows(RowArgs<T> a) {
         xm = xkk_of(em); x0 = xkk_of(e0); xp = xkk_of(ep);
       }
       const T xs = xm + x0;
-      T tuu = -xs * ux;                                           // compsgs1.f90:144-146
+      T tuu = -xs * ux;                                           // compsgs1.f90:41-43
       T tvv = -xs * vy;
       T tuv = -xs * (uy + vx) * T(0.5);
       const T wz0 = (w0 - wm) * dzm;                              // derivs: wz at level iz
       const T wzp = (wp - w0) * dzm;
-      T tww0 = -xs * wz0;                                         // compsgs1.f90:129,151
+      T tww0 = -xs * wz0;                                         // compsgs1.f90:26,48
       T twwp = -(x0 + xp) * wzp;
       T tuw0, tvw0, tuwm, tvwm;
       if (top) {                                                  // sufsim_upper
         wall_tau(a.ws, 1, u0, v0, tuw0, tvw0);
-      } else {                                                    // compsgs1.f90:152-153
+      } else {                                                    // compsgs1.f90:49-50
         tuw0 = -x0 * ((up - u0) * dzm + wx0);
         tvw0 = -x0 * ((vp - v0) * dzm + wy0);
       }
@@ -1162,7 +1162,7 @@ k_rows(RowArgs<T> a) {
         if (!top) tuw0 = tvw0 = T(0);
         if (iz != 1) tuwm = tvwm = T(0);
       }
-      // compadv1.f90:19-51 and compsgs3.f90:205-224
+      // compadv1.f90:19-51 and compsgs3.f90:40-59
       fa[0] = -(u0 * u0 + tuu);
       // the imposed gradient 0.016 (comp1.f90:152) is a (0,0)-mode constant: k_press adds it in FP64
       fb[0] = -((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
@@ -1176,7 +1176,7 @@ k_rows(RowArgs<T> a) {
         const T wa = T(0.5) * (wp + w0), wb = T(0.5) * (w0 + wm);
         Hw = -(wa * wa - wb * wb) * dzm - (twwp - tww0) * dzm;
         // SGS TKE: advection (compadv1.f90:59-64), diffusion and dissipation
-        // (compsgs3.f90:185-200)
+        // (compsgs3.f90:20-35)
         Fxe = -T(0.5) * e0 * (u0 + up) + T(2) * x0 * ex;
         Fye = -T(0.5) * e0 * (v0 + vp) + T(2) * x0 * ey;
         He = -(wp * ep - wm * em) * todzm
@@ -1246,7 +1246,7 @@ k_rows(RowArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // k_rows_mlp: k_rows with the learned closure in place of the compsgs1
-// stresses (no reference counterpart; insertion point compsgs1.f90:104-156).
+// stresses (no reference counterpart; insertion point compsgs1.f90:1-53).
 //
 // The network runs at every 3/2-grid cell centre.  Its inputs:
 //   - ux uy vx vy at the level;
@@ -1305,7 +1305,7 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
 
   auto xkk_of = [&](T e) {
     T v = T(0.1) * alk * sqrt(e);
-    return v < xkmax ? v : xkmax;   // complen.f90:89
+    return v < xkmax ? v : xkmax;   // complen.f90:21
   };
   // the network at a cell with gradients g -> stress t, accepted or not
   auto closure = [&](int k, const float (&g)[9], float (&t)[6]) -> bool {
@@ -1366,7 +1366,7 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
 
   // the six synthesis pairs of iteration iz: u, v at iz+2; w, e at face
   // iz+1; the horizontal gradients of u, v at iz+1 and of w at face iz+1;
-  // ex, ey at face iz (at iz = nnz-1 face nnz-2 again, derivs2.f90:279)
+  // ex, ey at face iz (at iz = nnz-1 face nnz-2 again, derivs2.f90:9,20-21)
   auto level_srcs = [&](int iz, const cx<T>** sa, const cx<T>** sb) {
     sa[0] = lvl(FU, iz + 2);  sb[0] = lvl(FV, iz + 2);
     sa[1] = face(FW, iz + 1); sb[1] = row(FE, clampF(iz + 1));
@@ -1428,7 +1428,7 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
     } else if (ok0 && ok1) {
       tuw0 = (T)(0.5f * (t0[4] + t1[4]));
       tvw0 = (T)(0.5f * (t0[5] + t1[5]));
-    } else {                                                      // compsgs1.f90:152-153
+    } else {                                                      // compsgs1.f90:49-50
       tuw0 = -x0 * ((up - u0) * dzm + wx0);
       tvw0 = -x0 * ((vp - v0) * dzm + wy0);
     }
@@ -1437,12 +1437,12 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
       double sums[7];
       {
         const T xs = xm + x0;
-        T tuu = -xs * ux0;                                        // compsgs1.f90:144-146
+        T tuu = -xs * ux0;                                        // compsgs1.f90:41-43
         T tvv = -xs * vy0;
         T tuv = -xs * (uy0 + vx0) * T(0.5);
         const T wz0 = (w0 - wm) * dzm;
         const T wzp = (wp - w0) * dzm;
-        T tww0 = -xs * wz0;                                       // compsgs1.f90:129,151
+        T tww0 = -xs * wz0;                                       // compsgs1.f90:26,48
         T twwp = -(x0 + xp) * wzp;
         if (ok0) {
           const float tr = (t0[0] + t0[1] + t0[2]) * (1.0f / 3.0f);
@@ -1459,7 +1459,7 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
           tuwm = fuw;
           tvwm = fvw;
         }
-        // compadv1.f90:19-51 and compsgs3.f90:205-224
+        // compadv1.f90:19-51 and compsgs3.f90:40-59
         fa[0] = -(u0 * u0 + tuu);
         // the imposed gradient 0.016 (comp1.f90:152) is a (0,0)-mode constant: k_press adds it in FP64
         fb[0] = -((up + u0) * w0 - (um + u0) * wm) * todzm - (tuw0 - tuwm) * dzm;
@@ -1558,7 +1558,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_rows_mlp(RowArgs<T> a, const mlp
 //
 // The network and its TKE fallback run at the cell centres of the small grid.
 // The inputs and the fallback formulas are those of k_rows_mlp, with K from
-// the physical e at the faces (compsgs1.f90:144-153).  The row kernel emits
+// the physical e at the faces (compsgs1.f90:41-50).  The row kernel emits
 // the stress-divergence contributions to the tendencies as x-spectral rows:
 //   G_u = i kx F^x + H^ and F^y, per component, with
 //   u: Fx = -t11, Fy = -t12, H = -(t13(iz) - t13(iz-1))/dz;
@@ -1779,7 +1779,7 @@ k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
   unsigned long long nacc = 0;
   auto xkk_of = [&](T e) {
     T v = T(0.1) * alk * sqrt(e);
-    return v < xkmax ? v : xkmax;   // complen.f90:89
+    return v < xkmax ? v : xkmax;   // complen.f90:21
   };
   auto closure = [&](int k, const float (&g)[9], float (&t)[6]) -> bool {
     const float zc = ((float)k - 0.5f) * dzf;
@@ -1880,14 +1880,14 @@ k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
       if (ok0 && ok1) {
         f13 = (T)(0.5f * (t0[4] + t1[4]));
         f23 = (T)(0.5f * (t0[5] + t1[5]));
-      } else {                                                    // compsgs1.f90:152-153
+      } else {                                                    // compsgs1.f90:49-50
         f13 = -x0 * ((up - u0) * dzm + wx0);
         f23 = -x0 * ((vp - v0) * dzm + wy0);
       }
     }
     if (iz >= z0) {
       const T xs = xm + x0;
-      T c11 = -xs * ux0, c22 = -xs * vy0, c12 = -xs * (uy0 + vx0) * T(0.5);   // compsgs1.f90:144-146
+      T c11 = -xs * ux0, c22 = -xs * vy0, c12 = -xs * (uy0 + vx0) * T(0.5);   // compsgs1.f90:41-43
       T c33 = -xs * (w0 - wm) * dzm;
       T c33p = -(x0 + xp) * (wp - w0) * dzm;
       if (ok0) {
@@ -2214,7 +2214,7 @@ struct PressArgs {
   double dz, dzm, dtab1, dtab2, dtabm;
 };
 
-// Reciprocal pivots of the Thomas elimination (tridv.f90:28-43) for every
+// Reciprocal pivots of the Thomas elimination (tridv.f90:34-43) for every
 // mode and level: d_1 = 1/diag_1, d_z = 1/(diag_z - d_z-1).  One thread per mode.
 __global__ void __launch_bounds__(64) k_press_pivots(double* piv, const double* kxd, const double* kyd,
                                                      int mx, int ny, int nz, double dz) {
@@ -2224,7 +2224,7 @@ __global__ void __launch_bounds__(64) k_press_pivots(double* piv, const double* 
   const int j = m / mx, kx = m - j * mx;
   const double kxp = kxd[kx], kyp = kyd[j];
   const double dz2 = dz * dz;
-  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:37-43)
+  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:33-42)
   double d_prev = 0.0;
   for (int z = 1; z <= nz; ++z) {
     const double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
@@ -2252,7 +2252,7 @@ __device__ void press_zero_mode(const PressArgs<T>& a) {
     }
     const double ru = (double)a.Rn[0][o].x + FORCING;   // comp1.f90:152
     const double rv = (double)a.Rn[1][o].x;
-    u += a.dtab1 * ru;                           // comp2.f90:35-36
+    u += a.dtab1 * ru;                           // comp2.f90:34-35
     v += a.dtab1 * rv;
     mu[z] = u; mv[z] = v; mru[z] = ru; mrv[z] = rv;
     a.U[0][o] = cx<T>{(T)u, T(0)};
@@ -2292,7 +2292,7 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   if (j == a.ny / 2) return;                    // Nyquist row: always zero
   const double kxp = a.kxd[kx], kyp = a.kyd[j];
   const double dz2 = a.dz * a.dz;
-  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:37-43)
+  const double k2 = kxp * kxp + kyp * kyp;      // xk2 (init.f90:33-42)
   const int nz = a.nz;
   const size_t pl = (size_t)modes;
   const bool sub = (a.istage != 1);
@@ -2306,7 +2306,7 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   const cx<T>* __restrict__ N1 = a.Rn[1];
   const cx<T>* __restrict__ N2 = a.Rn[2];
   cx<double>* __restrict__ sr = a.sr;
-  // forward sweep (comprhs.f90:17-35, press.F90:44-76, tridv.f90:28-43)
+  // forward sweep (comprhs.f90:17-35, press.F90:44-76, tridv.f90:34-43)
   cx<double> wm_prev = {0, 0}, rw_prev = {0, 0}, r_prev = {0, 0};
   double d_prev = 0.0;
   for (int zb = 1; zb <= nz; zb += ZU) {
@@ -2357,7 +2357,7 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
       r_prev = r;
     }
   }
-  // backward sweep (tridv.f90:44-53) fused with comp2 (comp2.f90:21-41)
+  // backward sweep (tridv.f90:44-53) fused with comp2 (comp2.f90:17-40)
   cx<double> p_next = {0, 0};
   for (int zt = nz; zt >= 1; zt -= ZU) {
     double ld[ZU];
@@ -2379,14 +2379,14 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
       if (z < 1) break;
       const size_t o = (size_t)z * pl + m;
       const cx<double> p = (z == nz) ? cscale(lr[q], ld[q]) : cscale(csub(lr[q], p_next), ld[q]);
-      const cx<double> ru = csub(widen(lru[q]), cmuli(cscale(p, kxp)));   // comp2.f90:21
-      const cx<double> rv = csub(widen(lrv[q]), cmuli(cscale(p, kyp)));   // comp2.f90:25
+      const cx<double> ru = csub(widen(lru[q]), cmuli(cscale(p, kxp)));   // comp2.f90:18-19
+      const cx<double> rv = csub(widen(lrv[q]), cmuli(cscale(p, kyp)));   // comp2.f90:23-24
       cx<double> rw = {0, 0}, wn = {0, 0};
       if (z != nz) {
         rw = csub(widen(lrw[q]), cscale(csub(p_next, p), a.dzm));        // comp2.f90:29
         wn = cadd(widen(lw[q]), cscale(rw, a.dtab1));                     // comp2.f90:37
       }
-      st2(U0 + o, cadd(widen(lu[q]), cscale(ru, a.dtab1)));              // comp2.f90:35-36
+      st2(U0 + o, cadd(widen(lu[q]), cscale(ru, a.dtab1)));              // comp2.f90:34-35
       st2(U1 + o, cadd(widen(lv[q]), cscale(rv, a.dtab1)));
       st2(U2 + o, wn);
       st2(R0 + o, ru);
@@ -2427,7 +2427,7 @@ __global__ void __launch_bounds__(256) k_shift_levels(double* out, const double*
 // k_erows: SGS TKE update on the small grid and its x-forward transform.
 // grid (NY/2, nz); each workgroup handles rows y and y + NY/2 of level z.
 //   mode 0: re <- x-inverse(Tre); e <- max(e - dtab2 re_old + dtab1 re, emin)
-//           (comp1.f90:131, comp2.f90:261); Ex <- x-forward(e)
+//           (comp1.f90:131, comp2.f90:38); Ex <- x-forward(e)
 //   mode 1: Ex <- x-forward(e) only (after set_state)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -2489,7 +2489,7 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
       T* pr0 = re + (size_t)y0 * NX + x;
       T* pr1 = re + (size_t)y1 * NX + x;
       T e0 = *pe0, e1 = *pe1;
-      if (z == nz - 1) {                       // upper: e(nnz) = e(nnz-1) (upperbc.f90:197-201)
+      if (z == nz - 1) {                       // upper: e(nnz) = e(nnz-1) (upperbc.f90:20-24)
         e[(size_t)plane + (size_t)y0 * NX + x] = e0;
         e[(size_t)plane + (size_t)y1 * NX + x] = e1;
       }
@@ -2628,7 +2628,7 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
 }
 
 // sums of the per-row partials: divz (comp1.f90:145) and the SGS means
-// (compsgs3.f90:236-243).  grid (slots), block 256
+// (compsgs3.f90:71-78).  grid (slots), block 256
 __global__ void k_reduce_part(const double* __restrict__ part, size_t pstride, int nyb, int q0, int nq,
                               double scale, double* out, int slots) {
   __shared__ double red[256];

@@ -2,7 +2,7 @@
 //
 // A pointwise MLP 11 -> 64 -> 64 -> 64 -> 6 with GELU (tanh form) after each
 // hidden layer.  No counterpart in the reference: it replaces the stresses of
-// compsgs1.f90:104-156 (see k_rows_mlp).  BF16 operands on
+// compsgs1.f90:1-53 (see k_rows_mlp).  BF16 operands on
 // v_mfma_f32_32x32x16_bf16 with FP32 accumulation, for inference, the
 // training forward and backward passes; Adam on FP32 master weights.
 //
@@ -566,12 +566,12 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const Net* __restrict__ gnet
 //   fz*(K+1), trapezoid over the fine faces fz*(K+1) -/+ fz/2, zero at the
 //   top wall), 4 e (cell centre), 5..10 uu vv ww uv uw vw;
 //   11..16 the fine grid's own SGS stresses t11 t22 t33 t12 t13 t23 at cell
-//   centres.  These follow the TKE closure (complen.f90:88-89,
-//   compsgs1.f90:144-153), with K = min(0.1 alk sqrt(e), xkmax) at faces:
+//   centres.  These follow the TKE closure (complen.f90:20-21,
+//   compsgs1.f90:41-50), with K = min(0.1 alk sqrt(e), xkmax) at faces:
 //   -(K(k-1)+K(k)) ux, ..., and the face stresses -K (uz + wx) averaged to
 //   the centre (the wall faces take the adjacent interior face's value).
 // Cell-centred w is (w(k-1)+w(k))/2 with w(0) = 0; cell-centred e is
-// (e(k-1)+e(k))/2 with e(0) = e(1) (lowerbc.f90:13).
+// (e(k-1)+e(k))/2 with e(0) = e(1) (lowerbc.f90:15).
 constexpr int NFILT = 17;
 struct FilterArgs {
   const double* u;

@@ -19,7 +19,7 @@ constexpr int NMEAN = 10;   // uxym vxym wxym uu vv ww uw vw uv engsbz
 
 // compmn per level iz (blockIdx.x + 1):
 // - ul = ((u_iz - <u_iz>) + (u_iz+1 - <u_iz+1>))/2, with u(nnz+1) = 0
-//   (upperbc.f90:189-191), and vl likewise;
+//   (upperbc.f90:12-14), and vl likewise;
 // - w at face iz;
 // - uu = <ul ul>, vv, ww = <w w>, uw = <ul w>, vw, uv;
 // - the level means of u, v, w;
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_spectra(const cx<T>* __restrict__ U, co
       if (m == 0) cr = ci = 0.0;
     }
     const double p = cr * cr + ci * ci;
-    const int kyw = j <= nx / 2 ? j : j - nx;            // waveN of calc_spectra.f90:100-106
+    const int kyw = j <= nx / 2 ? j : j - nx;            // waveN of calc_spectra.f90:105-112
     const int r = (int)rint(sqrt((double)(k * k + kyw * kyw)));
     const double wt = k == 0 ? 1.0 : 2.0;
     if (r <= nx / 2) atomicAdd(&bins[r], wt * p);

@@ -8,8 +8,8 @@
 // fields ever hold after deal (les.F90:185-190): compress truncates to that band
 // (filter_fftw.f90:300-303), xderiv zeroes the Nyquist terms
 // (filter_fftw.f90:182-183), and the pressure keeps no Nyquist modes
-// (press.F90:153-154).  The SGS TKE e is clamped pointwise in physical space
-// (comp2.f90:261) and so stays physical, with its x-spectral rows kept beside
+// (press.F90:109-110).  The SGS TKE e is clamped pointwise in physical space
+// (comp2.f90:38) and so stays physical, with its x-spectral rows kept beside
 // it for the next stage's synthesis.  Pressure p is a band spectrum in FP64.
 #include <hip/hip_runtime.h>
 
@@ -217,10 +217,10 @@ struct EngineT : Engine {
     HIPCHK(hipMemcpy(ctx->twNXB, t2.data(), NXB * sizeof(C), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tw64NX, t3.data(), NX * sizeof(cx<double>), hipMemcpyHostToDevice));
     std::vector<double> kx(MX), ky(NY);
-    for (int k = 0; k < MX; ++k) kx[k] = (double)k * 2.0 * PI / ctx->p.xl;   // init.f90:437-440
+    for (int k = 0; k < MX; ++k) kx[k] = (double)k * 2.0 * PI / ctx->p.xl;   // init.f90:24-27
     for (int j = 0; j < NY; ++j) {
       const int kk = (j < NY / 2) ? j : (j == NY / 2 ? 0 : j - NY);
-      ky[j] = (double)kk * 2.0 * PI / ctx->p.yl;                            // init.f90:441-444
+      ky[j] = (double)kk * 2.0 * PI / ctx->p.yl;                            // init.f90:28-31
     }
     std::vector<T> kxt(kx.begin(), kx.end()), kyt(ky.begin(), ky.end());
     HIPCHK(hipMemcpy(ctx->kxp, kxt.data(), MX * sizeof(T), hipMemcpyHostToDevice));
@@ -639,7 +639,7 @@ struct EngineT : Engine {
     if ((rc = to_band(ctx, u, cptr(ctx->U))) != CF_OK) return rc;
     if ((rc = to_band(ctx, v, cptr(ctx->V))) != CF_OK) return rc;
     if ((rc = to_band(ctx, w, cptr(ctx->W))) != CF_OK) return rc;
-    // w(nnz) is the top wall (upperbc.f90:27)
+    // w(nnz) is the top wall (upperbc.f90:26)
     HIPCHK(hipMemsetAsync(cptr(ctx->W) + (size_t)nz * NY * MX, 0, NY * MX * sizeof(C), s));
     // the plane means of u and v in FP64 (the (0,0) coefficients; deal keeps
     // them), the RK registers' means zero
@@ -1028,7 +1028,7 @@ int cf_compmn(cf_ctx* ctx, double* meanbuf) {
     mb[1] = l[9];                                                    // engsbz
     mb[2] = red[0 * ctx->slots + i + 1];                             // divz
     mb[12] = 0.0;                                                    // pple is never computed
-    // SGS means uusb vvsb uvsb wwsb uwsb vwsb (allocate.f90:289-294)
+    // SGS means uusb vvsb uvsb wwsb uwsb vwsb (allocate.f90:106-111)
     for (int r = 0; r < 6; ++r) mb[13 + r] = ctx->have_sgs ? red[(size_t)(r + 1) * ctx->slots + i + 1] : 0.0;
   }
   if (ctx->have_sgs && ctx->have_sred) {

@@ -399,7 +399,7 @@ def test_box_filter_matches_numpy(f):
         ks = np.arange(kf - fz // 2, kf + fz // 2 + 1)
         wf[K] = np.tensordot(wts, wxy[ks - 1], axes=1) / wts.sum()
     ref["wf"] = wf
-    # the fine grid's TKE-closure stresses (compsgs1.f90:144-153) at cell centres
+    # the fine grid's TKE-closure stresses (compsgs1.f90:41-50) at cell centres
     def ddx(a):
         F = np.fft.rfft2(a); kx = 2 * np.pi / pr.xl * np.arange(F.shape[-1]); kx[-1] = 0
         return np.fft.irfft2(F * 1j * kx, s=a.shape[-2:])

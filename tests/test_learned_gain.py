@@ -7,7 +7,7 @@ samples: 16384 cells of the last snapshot of the 512^2 x 256 FP32 run
 (tests/golden/sgs_heldout_512x256.npz, written by channelflow.learned).  The
 targets are the complete subfilter stresses: the resolved part plus the fine
 run's own SGS stress.  Baselines:
-- the TKE closure -2 K S with K = 0.1 dsl sqrt(e) (complen.f90:88-89);
+- the TKE closure -2 K S with K = 0.1 dsl sqrt(e) (complen.f90:20-21);
 - the same with its best least-squares coefficient;
 - no model.
 The CPU test uses the FP64 oracle network with BF16 emulation; the GPU test

@@ -1,5 +1,5 @@
 """Closed-form pins of the oracle's diagnostics: compmn (compmn.F90:17-62),
-the SGS stress means of an mnout stage (compsgs3.f90:139-146) and the x-y
+the SGS stress means of an mnout stage (compsgs3.f90:71-78) and the x-y
 spectra of calc_spectra (calc_spectra.f90:139-257).
 
 The expected values are derived by hand from constructed fields (plane means
@@ -74,7 +74,7 @@ def test_sgs_means_of_an_mnout_stage():
     """imodel 3, e_k = E_k + eps_k sin x, u_k = U_k + a_k cos x + s_k sin x,
     v = 0, w_k = b_k cos x + p_k sin x, so the 3/2-grid expanded fields equal
     these analytic functions at the 3/2-grid points.  The stress means are then
-    plane means of pointwise products there (sum * xymb, compsgs3.f90:139-146):
+    plane means of pointwise products there (sum * xymb, compsgs3.f90:71-78):
     uusb_k = <-(K_k-1 + K_k) ux_k>, wwsb_k = <-(K_k-1 + K_k) wz_k>,
     uwsb_k = <tau_uw at face k> (the Schumann stress at face nnz), uvsb = vvsb =
     vwsb = 0, with K_f = min(0.1 dsl sqrt(e_f), xkmax) (complen.f90:21)."""

@@ -43,7 +43,7 @@ def test_sgs_stresses_match_reference(harness, tmp_path, imodel, isurface):
     reference gives windm = 0.01 there, as a NaN ufree does under a > b ? a : b,
     so the oracle runs with ufree = NaN for this comparison; the product
     takes ufree = 0.  Case 2 is not compared: it also reads ufluc/vfluc, which
-    the reference computes only for isurface > 2 (lowerbc.f90:95-99,131-132);
+    the reference computes only for isurface > 2 (lowerbc.f90:95-99,132-133);
     test_oracle_wall.py pins it by invariants."""
     pr = O.Params(16, 16, 8, dt=0.01, imodel=imodel, isurface=isurface,
                   ufree=float("nan") if isurface in (2, 4, 5) else 0.0)

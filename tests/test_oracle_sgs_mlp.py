@@ -265,7 +265,7 @@ def test_small_grid_all_fallback_equals_tke_on_the_small_grid():
 
 def test_small_grid_stresses_by_hand_at_one_point():
     """The TKE fallback stresses of the small-grid mode at one point, from the
-    formulas of compsgs1.f90:144-153 written out directly."""
+    formulas of compsgs1.f90:41-50 written out directly."""
     pr = O.Params(16, 16, 8, dt=0.01)
     u, v, w, e = O.initial_fields(pr)
     st = O.set_state(pr, u, v, w, e)

@@ -65,7 +65,7 @@ def test_y_reflection_equivariance():
 def test_momentum_budget_telescopes():
     """sum_k <ru(k)> = 0.016 nnz - (<tau_uw(nnz)> - <tau_uw(0)>)/dz: the
     advective z-fluxes vanish at the walls (w = 0) and the SGS z-fluxes leave
-    only the two wall stresses (Schumann, lowerbc.f90:129 / upperbc.f90:304)."""
+    only the two wall stresses (Schumann, lowerbc.f90:129 / upperbc.f90:127)."""
     pr, u, v, w, e = base(horizontally_uniform_e=False)
     st = O.set_state(pr, u, v, w, e)
     O.lower(st, pr); O.upper(st, pr)
@@ -124,7 +124,7 @@ def test_rk3_invariants():
 
 
 def test_rk3_is_third_order_on_linear_decay():
-    """The low-storage scheme (les.F90:26-27, comp1.f90:126, comp2.f90:257)
+    """The low-storage scheme (les.F90:26-27, comp1.f90:126, comp2.f90:34)
     applied to dphi/dt = lam phi has error O(dt^4) per step."""
     lam = -1.7
     errs = []

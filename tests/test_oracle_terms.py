@@ -1,10 +1,10 @@
 """Closed-form pins of the oracle's comp1 terms that the symmetry and budget
 tests leave open: the w advection and the TKE fluxes of compadv1
 (compadv1.f90:45-51, :59-63), the TKE diffusion and the w-equation stress
-divergence of compsgs3 (compsgs3.f90:88-98, :124-127), the stale ex/ey at face
-nnz-1 (derivs2.f90:157-169: ex, ey are not recomputed once iz >= nnz-1), and
+divergence of compsgs3 (compsgs3.f90:23-30, :56-59), the stale ex/ey at face
+nnz-1 (derivs2.f90:9-21: ex, ey are not recomputed once iz >= nnz-1), and
 the u/v-equation z fluxes with the wall stresses at both walls
-(compadv1.f90:22-24, compsgs3.f90:108-110, compsgs1.f90:41-50).
+(compadv1.f90:22-24, compsgs3.f90:40-42, compsgs1.f90:41-50).
 
 Each test builds fields that are low trigonometric polynomials in x (or y)
 with level-dependent amplitudes, so that every product stays inside the
@@ -177,7 +177,7 @@ def test_tke_diffusion_and_stale_gradient(d, profile):
     """imodel 4 with u = U_k uniform per level: the face shear is uz_k^2 alone,
     so K_k = min((0.1 dsl)^2 |U_k+1 - U_k|/dz, xkmax) varies with the face and
     not in x or y (complen.f90:13-23; uz(0) = uz(1), uz(nnz) = uz(nnz-1),
-    derivs1.f90:37, derivs2.f90:175).  e_k = E_k + eps_k T, no dissipation
+    derivs1.f90:37, derivs2.f90:27).  e_k = E_k + eps_k T, no dissipation
     (ceps = 0), v = w = 0:
 
     re_k = 0.5 kap eps_k (U_k + U_k+1) sinT [d = x]
@@ -185,7 +185,7 @@ def test_tke_diffusion_and_stale_gradient(d, profile):
            + dz^-2 [(K_k+1 + K_k)(e_k+1 - e_k) - (K_k + K_k-1)(e_k - e_k-1)]
 
     where eps'_k = eps_k, except at face nnz-1, where ex and ey are the ones of
-    face nnz-2 (derivs2.f90:157-169 computes them only for iz < nnz-1):
+    face nnz-2 (derivs2.f90:9-21 computes them only for iz < nnz-1):
     eps'_nnz-1 = eps_nnz-2.  The 'only_top' profile puts the horizontal
     variation of e at face nnz-1 alone (its horizontal diffusion must vanish);
     'only_below_top' puts it at face nnz-2 alone (face nnz-1 must still see it)."""
