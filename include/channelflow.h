@@ -143,6 +143,17 @@ int cf_compmn(cf_ctx* ctx, double* meanbuf);
 int cf_calc_spectra(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* hats, double* var,
                     double* lscales);
 
+/* Allocation-free variant of cf_calc_spectra.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as cf_calc_spectra, for at most 64 levels
+ * (CF_E_ARG beyond).  It calls no hipMalloc/hipFree: its scratch (ring sums,
+ * normalised outputs and the level list, about 0.8 MB) is a fixed device
+ * array held by each GPU until the process ends and shared by every call on
+ * that GPU, so calls that overlap on one GPU race. */
+int cf_calc_spectra_noalloc(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* hats, double* var,
+                            double* lscales);
+
 /* Time nsteps steps with HIP events on the context stream.  Writes the total
  * milliseconds and, when kernel_ms is not NULL, the average duration of each
  * kernel family of one step measured with events bracketing every launch

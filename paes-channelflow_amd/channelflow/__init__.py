@@ -45,6 +45,8 @@ EXPORTS = (
     "cf_mlp_put_samples", "cf_mlp_loss_grad", "cf_mlp_train", "cf_mlp_reset_optimizer",
     "cf_mlp_last_error", "cf_set_closure", "cf_get_closure_counts", "cf_filter_snapshot",
     "cf_calc_spectra", "cf_set_closure_grid",
+    # allocation-free variant (no reference counterpart)
+    "cf_calc_spectra_noalloc",
 )
 CF_CLOSURE_GRID_DEALIASED = 0
 CF_CLOSURE_GRID_SMALL = 1
@@ -128,6 +130,7 @@ def load_library(path: str = LIB_PATH):
     lib.cf_get_closure_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.cf_filter_snapshot.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp]
     lib.cf_calc_spectra.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), dp, dp, dp]
+    lib.cf_calc_spectra_noalloc.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), dp, dp, dp]
     lib.cf_set_closure_grid.argtypes = [vp, ctypes.c_int32]
     for name in EXPORTS:
         if name not in ("cf_kernel_name", "cf_device_bytes", "cf_last_error", "cf_mlp_last_error"):
@@ -279,14 +282,21 @@ class ChannelFlow:
         (hats, var, lscales) in the layout of the reference's uhat(r, iz):
         hats[f] is (nnx/2+1, nlev) for f = u, v, w; var[f] (nlev,);
         lscales (l111, l112, l221, l222, l331, l332), each (nlev,)."""
+        return self._spectra(self.lib.cf_calc_spectra, zlevels)
+
+    def calc_spectra_noalloc(self, zlevels):
+        """calc_spectra through cf_calc_spectra_noalloc (no device allocation;
+        at most 64 levels); same results."""
+        return self._spectra(self.lib.cf_calc_spectra_noalloc, zlevels)
+
+    def _spectra(self, fn, zlevels):
         zl = np.ascontiguousarray(zlevels, dtype=np.int32)
         nl = zl.size
         nb = self.params.nnx // 2 + 1
         h = np.empty((3, nl, nb))
         var = np.empty((3, nl))
         ls = np.empty((6, nl))
-        self._check(self.lib.cf_calc_spectra(self._h, nl, zl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                             _ptr(h), _ptr(var), _ptr(ls)))
+        self._check(fn(self._h, nl, zl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _ptr(h), _ptr(var), _ptr(ls)))
         return [h[f].T.copy() for f in range(3)], [var[f] for f in range(3)], [ls[q] for q in range(6)]
 
     def set_closure(self, net: "SgsNet | None"):

@@ -129,4 +129,43 @@ __global__ void __launch_bounds__(256) k_spectra(const cx<T>* __restrict__ U, co
   for (int r = tid; r < nb + 3; r += 256) o[r] = bins[r];
 }
 
+// calc_spectra's normalisation on the device, for one snapshot
+// (calc_spectra.f90:238-257 with fcounter = 1): in holds k_spectra's ring
+// sums per (level, field), [nlev][3][nb+3]; out receives hats[3][nlev][nb],
+// var[3][nlev] and lscales[6][nlev] (l_f1, l_f2 per field).  The ring
+// populations nCount of one (field, level) are counted here over the r2c half
+// plane kx = 0..nx/2 (weight 2 for 0 < kx < nx/2), every ky
+// (calc_spectra.f90:150-166).  One workgroup of 256; nx <= 512.
+__global__ void __launch_bounds__(256) k_spectra_finish(const double* __restrict__ in, int nlev, int nx,
+                                                        int ny, double pi, double* __restrict__ out) {
+  __shared__ double cnt[512 / 2 + 1];
+  const int nb = nx / 2 + 1, tid = threadIdx.x;
+  for (int r = tid; r < nb; r += 256) cnt[r] = 0.0;
+  __syncthreads();
+  for (int i = tid; i < nb * ny; i += 256) {
+    const int kx = i % nb, j = i / nb;
+    const int ky = j <= nx / 2 ? j : j - nx;
+    const int r = (int)rint(sqrt((double)(kx * kx + ky * ky)));
+    if (r <= nx / 2) atomicAdd(&cnt[r], (kx >= 1 && kx <= nx / 2 - 1) ? 2.0 : 1.0);
+  }
+  __syncthreads();
+  double* hats = out;
+  double* var = out + (size_t)3 * nlev * nb;
+  double* ls = var + (size_t)3 * nlev;
+  for (int i = tid; i < 3 * nlev * nb; i += 256) {
+    const int r = i % nb, fl = i / nb, l = fl % nlev, f = fl / nlev;
+    const double* b = in + ((size_t)l * 3 + f) * (nb + 3);
+    // the r = 0 bin is not normalised (calc_spectra.f90:239)
+    hats[i] = r == 0 ? b[0] : b[r] * pi * ((r + 0.5) * (r + 0.5) - (r - 0.5) * (r - 0.5)) / cnt[r];
+  }
+  for (int i = tid; i < 3 * nlev; i += 256) {
+    const int l = i % nlev, f = i / nlev;
+    const double* b = in + ((size_t)l * 3 + f) * (nb + 3);
+    const double v = b[nb + 2];
+    var[i] = v;
+    ls[(size_t)(2 * f) * nlev + l] = b[nb] * pi / v;           // calc_spectra.f90:252-257
+    ls[(size_t)(2 * f + 1) * nlev + l] = b[nb + 1] * pi / v;
+  }
+}
+
 }  // namespace cf

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,25 @@
 #include "cf_stats.hpp"
 
 using cf::cx;
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point cf_calc_spectra_noalloc: a
+// file-scope device array of fixed size.  Each GPU the library runs on holds
+// its own copy until the process ends; every call on a GPU shares that GPU's
+// copy (calls that overlap on one GPU race).  Host code uses the address that
+// hipGetSymbolAddress returns for the current device (scratch_ptr), never the
+// array's name, which on the host is only a stand-in.
+// Layout: k_spectra's ring sums [nlev][3][nb+3], then the normalised outputs
+// hats[3][nlev][nb], var[3][nlev], lscales[6][nlev], then the level list.
+// ---------------------------------------------------------------------------
+constexpr int kSpecMaxLevels = 64;
+constexpr int kSpecMaxBins = 512 / 2 + 1;                                      // nx/2 + 1, nx <= 512
+constexpr size_t kSpecRing = (size_t)kSpecMaxLevels * 3 * (kSpecMaxBins + 3);
+constexpr size_t kSpecOut = (size_t)kSpecMaxLevels * (3 * kSpecMaxBins + 3 + 6);
+constexpr size_t kSpecScratch = kSpecRing + kSpecOut + kSpecMaxLevels / 2;   // + nlev int32 levels
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kSpecScratch];
 
 namespace {
 
@@ -41,6 +61,32 @@ thread_local std::string g_create_error;
   } while (0)
 
 constexpr double PI = 3.14159265358979323846264338327950288;
+
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Looks up this device's copy of scratch once per device; the device must be current.
+int scratch_setup(int device, std::string& err) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    err = "device ordinal beyond the scratch table";
+    return CF_E_ARG;
+  }
+  if (g_scratch_ptr[device]) return CF_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return CF_E_HIP;
+  }
+  g_scratch_ptr[device] = (double*)ptr;
+  return CF_OK;
+}
+
+double* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
 
 const char* kKernelNames[CF_NKERN] = {"k_wall",        "k_col_synth", "k_divz_band",    "k_rows",
                                       "k_col_anal",    "k_sgs_cols_in", "k_sgs_rows", "k_sgs_cols_out",
@@ -850,7 +896,13 @@ int cf_create(const cf_params* p, cf_ctx** out) {
       ctx->ncu <= 0)
     ctx->ncu = 256;
   ctx->eng = ctx->fp32 ? make_engine_t<float>(nx) : make_engine_t<double>(nx);
-  int rc = ctx->eng->alloc(ctx);
+  int rc = scratch_setup(p->device, ctx->err);
+  if (rc != CF_OK) {
+    g_create_error = ctx->err;
+    cf_destroy(ctx);
+    return rc;
+  }
+  rc = ctx->eng->alloc(ctx);
   if (rc != CF_OK) {
     g_create_error = ctx->err;
     cf_destroy(ctx);
@@ -1085,6 +1137,42 @@ int cf_calc_spectra(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* h
       lscales[(size_t)(2 * f) * nlev + l] = b[nb] * PI / v;          // l_f1
       lscales[(size_t)(2 * f + 1) * nlev + l] = b[nb + 1] * PI / v;  // l_f2
     }
+  return CF_OK;
+}
+
+int cf_calc_spectra_noalloc(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, double* hats, double* var,
+                            double* lscales) {
+  if (!ctx || nlev <= 0 || nlev > ctx->nz || !zlevels || !hats || !var || !lscales) return CF_E_ARG;
+  if (nlev > kSpecMaxLevels) {
+    ctx->err = "cf_calc_spectra_noalloc takes at most " + std::to_string(kSpecMaxLevels) + " levels";
+    return CF_E_ARG;
+  }
+  for (int i = 0; i < nlev; ++i)
+    if (zlevels[i] < 1 || zlevels[i] > ctx->nz - 1) {
+      ctx->err = "calc_spectra levels must lie in 1..nnz-1";
+      return CF_E_ARG;
+    }
+  HIPCHK(hipSetDevice(ctx->p.device));
+  double* base = scratch_ptr(ctx->p.device);
+  if (!base) {
+    ctx->err = "no scratch address for this device";
+    return CF_E_HIP;
+  }
+  const int nb = ctx->nx / 2 + 1;
+  double* ring = base;
+  double* outp = base + kSpecRing;
+  int32_t* dz = (int32_t*)(base + kSpecRing + kSpecOut);
+  HIPCHK(hipMemcpyAsync(dz, zlevels, nlev * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+  int rc = ctx->eng->spectra(ctx, dz, nlev, ring);            // writes every ring sum it reads back
+  if (rc != CF_OK) return rc;
+  hipLaunchKernelGGL(cf::k_spectra_finish, dim3(1), dim3(256), 0, ctx->stream, ring, nlev, ctx->nx, ctx->ny,
+                     PI, outp);
+  HIPCHK(hipGetLastError());
+  const size_t nh = (size_t)3 * nlev * nb, nv = (size_t)3 * nlev, nl = (size_t)6 * nlev;
+  HIPCHK(hipMemcpyAsync(hats, outp, nh * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(var, outp + nh, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(lscales, outp + nh + nv, nl * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return CF_OK;
 }
 
