@@ -5,24 +5,30 @@ Metric: grid-point RK3 steps per second (nnx*nny*nnz*steps / time) on the
 256x256x128 channel at Re_tau = 395 (z0 = exp(-0.4*5)/395), seeded log-law +
 0.1 u_tau noise.  ``value`` is the FP32 path with the learned SGS closure:
 - FP32 fields, transforms, products and SGS;
-- the BF16-MFMA network at every 3/2-grid cell, with TKE fallback;
-- FP64 Poisson RHS, tridiagonal solve, projection and update.
+- the BF16-MFMA network at every nnx x nny cell centre (the grid its training
+  features live on; --closure-grid dealiased puts it on the 3/2 grid), with
+  the TKE closure as fallback;
+- FP64 Poisson RHS, tridiagonal solve, projection and update, and FP64 plane
+  means of u, v and their RK registers with the imposed gradient.
 The network is the trained one shipped in paes-channelflow_amd/weights/.
-Timed beside it: the FP64 plain path (TKE closure) and the FP32 path with
-the TKE closure.  Fields are resident in HBM before the timed region; the
-timed region is K full steps on the library stream bracketed by stream
-synchronisation, measured with HIP events.
+Timed beside it: the FP64 plain path (TKE closure), the FP32 path with the
+TKE closure and the learned path with the network on the other grid.
+Fields are resident in HBM before the timed region; the timed region is
+exactly K full steps on the library stream bracketed by stream
+synchronisation, measured with HIP events.  ``sustained`` then times 500
+more steps of the same path, long enough for an outside clock to see.
 
-One JSON line is printed.  ``roofline`` prices the dominant kernel (the
-row-tendency kernel k_rows): the bytes its launch must move -- the 8 y-synthesised
-x-spectral input rows and the 8 flux output rows on the 3/2 grid, each read or
-written once, 24 * N * sizeof(real) per launch for N = nnx*nny*nnz -- divided by
-its mean launch time from HIP events on the library stream, against the 8 TB/s
-HBM3E peak.  ``step_roofline`` prices the whole step at the compulsory bytes of
-the persistent arrays (FP32 path: 85 FP32 + 12 FP64 array passes = 436 B per
-grid-point step; FP64 path: 97 x 8 = 776 B).  ``cpu_baseline`` times the FP64
-numpy oracle (the CPU restatement of the reference stage) on one whole RK3
-step of the same 256x256x128 workload, on one host core.
+One JSON line is printed.  ``roofline`` prices the path's dominant kernel
+(the one with the longest launch, k_sgs_rows by default): the bytes its
+launch must move (its input and output rows, each read or written once) over
+its mean launch time from HIP events on the library stream, against the
+8 TB/s HBM3E peak; ``roofline.valu`` gives the same kernel against the VALU
+issue peak (instructions) and its VALU busy cycles, from counter profiles.
+``step_roofline`` prices the whole step at the compulsory bytes of the
+persistent arrays (FP32 path: 85 FP32 + 12 FP64 array passes = 436 B per
+grid-point step; FP64 path: 97 x 8 = 776 B).  ``cpu_baseline`` times the
+FP64 numpy oracle (the CPU restatement of the reference stage) on one whole
+RK3 step of the same 256x256x128 workload, on one host core.
 """
 from __future__ import annotations
 
@@ -101,6 +107,12 @@ MEASURED_VALU = {
     ("k_rows_mlp", 256, 128, "fp32"): 738.28e6,
     ("k_sgs_rows", 256, 128, "fp32"): 294.34e6,
 }
+# SQ_ACTIVE_INST_VALU per dispatch (4-cycle issue slots the waves spent in
+# VALU instructions; a transcendental takes two) where profiled
+# (profiles/sq_counters_learned_256x256x128.md).
+MEASURED_VALU_ACTIVE = {
+    ("k_sgs_rows", 256, 128, "fp32"): 346.64e6,
+}
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
 
@@ -111,8 +123,12 @@ def valu_roofline(kernel, args, ms):
     if insts is None:
         return None
     achieved = insts / (ms * 1e-3)
-    return {"achieved": achieved, "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
-            "frac": achieved / VALU_PEAK, "insts_per_launch": insts}
+    out = {"achieved": achieved, "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+           "frac": achieved / VALU_PEAK, "insts_per_launch": insts}
+    active = MEASURED_VALU_ACTIVE.get((kernel, args.nx, args.nz, "fp32"))
+    if active is not None:
+        out["busy_frac"] = active / (ms * 1e-3) / VALU_PEAK
+    return out
 
 
 def parse():
@@ -133,12 +149,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=0,
                     help="levels of the CPU-baseline sample (default: the full nz)")
+    ap.add_argument("--sustain", type=int, default=500,
+                    help="steps of the headline path timed once more after the K steps (0: none)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device copy of 256 MiB (counter calibration under rocprofv3)")
     return ap.parse_args()
 
 
-def run_gpu(args, precision_name, learned=False, grid=None):
+def run_gpu(args, precision_name, learned=False, grid=None, sustain=0):
     import numpy as np
     import channelflow as C
     nx, nz = args.nx, args.nz
@@ -163,13 +181,19 @@ def run_gpu(args, precision_name, learned=False, grid=None):
     t0 = time.perf_counter()
     ms = g.time_steps(args.steps)
     wall = time.perf_counter() - t0
+    sustained = None
+    if sustain:
+        t1 = time.perf_counter()
+        ms_s = g.time_steps(sustain)
+        sustained = {"steps": sustain, "ms_per_step": ms_s / sustain,
+                     "host_wall_s": time.perf_counter() - t1}
     _, per_kernel = g.time_steps(1, per_kernel=True)
     g.sync()
     st = g.get_state()
     finite = all(bool(np.isfinite(st[k]).all()) for k in "uvwe")
     acc = g.closure_counts() if learned else (0, 0)
     out = dict(ms=ms, wall=wall, per_kernel=per_kernel, dt=dt, z0=z0, finite=finite,
-               device_bytes=g.device_bytes(), accepted=acc)
+               device_bytes=g.device_bytes(), accepted=acc, sustained=sustained)
     g.close()
     if net is not None:
         net.close()
@@ -229,7 +253,7 @@ def main():
         torch.cuda.synchronize()
         print("[bench] copy probe: 256 MiB read + 256 MiB written", file=sys.stderr, flush=True)
     print(f"[bench] fp32 + learned closure {args.nx}x{args.nx}x{args.nz}", file=sys.stderr, flush=True)
-    r32 = run_gpu(args, "fp32", learned=True)
+    r32 = run_gpu(args, "fp32", learned=True, sustain=args.sustain)
     value = pts * args.steps / (r32["ms"] * 1e-3)
     other = "dealiased" if args.closure_grid == "small" else "small"
     ro = None
@@ -297,6 +321,7 @@ def main():
             "frac": step_achieved / HBM_PEAK_GBS,
             "basis": "436 compulsory HBM bytes per grid-point RK3 step x grid-pt steps/s",
         },
+        "sustained": r32["sustained"],
         "kernel_ms_per_stage": kern,
         "kernel_gbs": per_kernel_gbs,
         "fp64_plain": None,

@@ -89,6 +89,11 @@ constexpr int span_before(int n, int s) {
 // stride-R writes of a first stage spread over the banks.  Element i of a
 // transform sits at pidx(i); transforms are padlen(N) elements apart.
 constexpr int padlen(int n) { return n + n / 8; }
+// Stride of the column tiles' transforms: at least padlen(n) and = 8 (mod 32)
+// complex elements.  A tile's global rows are copied in and out with the 8
+// columns of a row on consecutive lanes; a stride of 0 or 16 (mod 32) would
+// put those 8 lanes on one LDS bank, this one spreads them over the banks.
+constexpr int colpad(int n) { return padlen(n) + ((8 - padlen(n) % 32) + 32) % 32; }
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
 
 template <typename T, int R, bool INV>
@@ -184,7 +189,7 @@ __device__ __forceinline__ void fft_sync() {
 // One Stockham stage over NBT transforms of length N held at a[t*N + n],
 // executed by NT threads (a whole workgroup, or one wavefront of 64 lanes
 // when WAVE).  Entry and exit are synchronised at that scope.
-template <typename T, int N, bool INV, int NBT, int NT, int S, bool WAVE>
+template <typename T, int N, bool INV, int NBT, int NT, int S, bool WAVE, int PLD = padlen(N)>
 __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
   constexpr int R = radix_of(N, S);
   constexpr int NS = span_before(N, S);
@@ -199,7 +204,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
     if ((TOT % NT) == 0 || idx < TOT) {
       const int t = idx / NBF;
       const int j = idx - t * NBF;
-      const cx<T>* src = a + t * padlen(N);
+      const cx<T>* src = a + t * PLD;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * NBF)];
       if constexpr (NS > 1) {
@@ -221,7 +226,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
       const int t = idx / NBF;
       const int j = idx - t * NBF;
       const int k = j % NS;
-      cx<T>* dst = a + t * padlen(N);
+      cx<T>* dst = a + t * PLD;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) dst[pidx(base + r * NS)] = v[q][r];
@@ -230,25 +235,26 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
   fft_sync<WAVE>();
 }
 
-template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false>
+template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N)>
 __device__ __forceinline__ void fft_run(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
   if constexpr (S < n_stages(N)) {
-    fft_stage<T, N, INV, NBT, NT, S, WAVE>(a, tw, tid);
-    fft_run<T, N, INV, NBT, NT, S + 1, WAVE>(a, tw, tid);
+    fft_stage<T, N, INV, NBT, NT, S, WAVE, PLD>(a, tw, tid);
+    fft_run<T, N, INV, NBT, NT, S + 1, WAVE, PLD>(a, tw, tid);
   }
 }
 
-// NBT transforms (padded layout, see padlen) spread over the NW wavefronts of a workgroup, each wave
-// running its share with wave-level ordering only.  The caller synchronises
-// the workgroup before (inputs written) and after (outputs read by others).
-template <typename T, int N, bool INV, int NBT, int NW>
+// NBT transforms (PLD elements apart, padded inside as pidx) spread over the
+// NW wavefronts of a workgroup, each wave running its share with wave-level
+// ordering only.  The caller synchronises the workgroup before (inputs
+// written) and after (outputs read by others).
+template <typename T, int N, bool INV, int NBT, int NW, int PLD = padlen(N)>
 __device__ __forceinline__ void fft_waves(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
   const int w = tid >> 6, lane = tid & 63;
   if constexpr (NBT % NW == 0) {
     constexpr int PW = NBT / NW;
-    fft_run<T, N, INV, PW, 64, 0, true>(a + w * PW * padlen(N), tw, lane);
+    fft_run<T, N, INV, PW, 64, 0, true, PLD>(a + w * PW * PLD, tw, lane);
   } else {
-    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true>(a + t * padlen(N), tw, lane);
+    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true, PLD>(a + t * PLD, tw, lane);
   }
 }
 

@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TKC, NT = Sh::NT;
-  constexpr int PLB = padlen(NYB), PLS = padlen(NY);
+  constexpr int PLB = colpad(NYB), PLS = colpad(NY);
   constexpr int LPS = (TK * NY + NT - 1) / NT;        // band values per thread
   constexpr int NZP = NYB - NY + 1;                    // zero modes NY/2 .. NYB - NY/2
   static_assert(PLS <= PLB, "the length-NY batch fits the length-NYB batch");
@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
       buf[c * PLS + pidx(y)] = src[(size_t)y * MX + c];
     }
     __syncthreads();
-    fft_waves<T, NY, false, TK, NT / 64>(buf, tws, tid);
+    fft_waves<T, NY, false, TK, NT / 64, PLS>(buf, tws, tid);
     __syncthreads();
     const T s = T(1) / T(NY);
 #pragma unroll
@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
       buf[c * PLB + pidx(NY / 2 + (i - c * NZP))] = cx<T>{0, 0};
     }
     __syncthreads();
-    fft_waves<T, NYB, true, TK, NT / 64>(buf, twb, tid);
+    fft_waves<T, NYB, true, TK, NT / 64, PLB>(buf, twb, tid);
     __syncthreads();
     cx<T>* od = o + (size_t)d * a.bstride;
 #pragma unroll
@@ -792,7 +792,7 @@ __global__ void __launch_bounds__(3 * NX / 2) k_eclamp_rows(EclampArgs<T> a) {
 
 template <typename T, int NX>
 __global__ void __launch_bounds__(256) k_eclamp_cols(EclampArgs<T> a) {
-  constexpr int NXB = 3 * NX / 2, NYB = NXB, P = 8, NT = 256, PL = padlen(NYB);
+  constexpr int NXB = 3 * NX / 2, NYB = NXB, P = 8, NT = 256, PL = colpad(NYB);
   static_assert(NXB % (2 * P) == 0, "column pairs per workgroup");
   if (!a.dirty[0]) return;
   __shared__ cx<T> buf[P * PL];
@@ -810,7 +810,7 @@ __global__ void __launch_bounds__(256) k_eclamp_cols(EclampArgs<T> a) {
       buf[q * PL + pidx(yy)] = cx<T>{r[0], r[1]};
     }
     __syncthreads();
-    fft_waves<T, NYB, false, P, NT / 64>(buf, tw, tid);
+    fft_waves<T, NYB, false, P, NT / 64, PL>(buf, tw, tid);
     __syncthreads();
     for (int i = tid; i < P * NYB; i += NT) {
       const int q = i / NYB, k = i % NYB;
@@ -819,7 +819,7 @@ __global__ void __launch_bounds__(256) k_eclamp_cols(EclampArgs<T> a) {
       buf[q * PL + pidx(k)] = cx<T>{-kk * z.y, kk * z.x};
     }
     __syncthreads();
-    fft_waves<T, NYB, true, P, NT / 64>(buf, tw, tid);
+    fft_waves<T, NYB, true, P, NT / 64, PL>(buf, tw, tid);
     __syncthreads();
     const T s = T(1) / T(NYB);
     for (int i = tid; i < P * NYB; i += NT) {
@@ -1974,7 +1974,7 @@ struct SgsColsInArgs {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = padlen(NY);
+  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = colpad(NY);
   const int f = blockIdx.z, z = blockIdx.y + 1;
   if (f == 2 && z >= a.nz) return;                 // w lives on faces 1..nz-1
   const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK, tid = threadIdx.x;
@@ -1991,7 +1991,7 @@ __global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
     buf[(TK + c) * PL + pidx(j)] = cx<T>{-k * v.y, k * v.x};
   }
   __syncthreads();
-  fft_waves<T, NY, true, 2 * TK, NT / 64>(buf, tw, tid);
+  fft_waves<T, NY, true, 2 * TK, NT / 64, PL>(buf, tw, tid);
   __syncthreads();
   const size_t off = (size_t)z * NY * MX + k0;
   cx<T>* o0 = a.S + (size_t)f * a.sstride + off;
@@ -2020,7 +2020,7 @@ struct SgsColsOutArgs {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = padlen(NY);
+  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = colpad(NY);
   const int cmp = blockIdx.z, z = blockIdx.y + 1;
   if (cmp == 2 && z >= a.nz) return;               // rw(nnz) = 0
   const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK, tid = threadIdx.x;
@@ -2036,7 +2036,7 @@ __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
     buf[(d * TK + c) * PL + pidx(y)] = (d ? fy : g)[(size_t)y * MX + c];
   }
   __syncthreads();
-  fft_waves<T, NY, false, 2 * TK, NT / 64>(buf, tw, tid);
+  fft_waves<T, NY, false, 2 * TK, NT / 64, PL>(buf, tw, tid);
   __syncthreads();
   const T s = T(1) / T(NY);
   cx<T>* R = a.R[cmp] + off;
@@ -2079,7 +2079,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TKC, NT = Sh::NT;
-  constexpr int PLB = padlen(NYB), PLS = padlen(NY);
+  constexpr int PLB = colpad(NYB), PLS = colpad(NY);
   constexpr int NKT = MX / TK;
   constexpr int LPT = (TK * NYB + NT - 1) / NT;      // tile elements per thread
   constexpr int OPT = (TK * NY + NT - 1) / NT;       // band outputs per thread
@@ -2122,7 +2122,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
     if (FULL || idx < TK * NYB) buf[(idx % TK) * PLB + pidx(idx / TK)] = lg[i];
   }
   __syncthreads();
-  fft_waves<T, NYB, false, TK, NT / 64>(buf, twb, tid);
+  fft_waves<T, NYB, false, TK, NT / 64, PLB>(buf, twb, tid);
   __syncthreads();
   cx<T> v[OPT];                              // G, then the band result
 #pragma unroll
@@ -2141,7 +2141,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
     if (FULL || idx < TK * NYB) buf[(idx % TK) * PLB + pidx(idx / TK)] = lf[i];
   }
   __syncthreads();
-  fft_waves<T, NYB, false, TK, NT / 64>(buf, twb, tid);
+  fft_waves<T, NYB, false, TK, NT / 64, PLB>(buf, twb, tid);
   __syncthreads();
   const T s = T(1) / T(NYB);
 #pragma unroll
@@ -2174,7 +2174,7 @@ __global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
     if (i < TK * NY) buf[(i % TK) * PLS + pidx(i / TK)] = v[q];
   }
   __syncthreads();
-  fft_waves<T, NY, true, TK, NT / 64>(buf, tws, tid);
+  fft_waves<T, NY, true, TK, NT / 64, PLS>(buf, tws, tid);
   __syncthreads();
   cx<T>* o = a.Tre + (size_t)z * splane + k0;
 #pragma unroll
@@ -2599,7 +2599,7 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
                                                      const cx<T>* twg, int l0) {
   constexpr int MX = NX / 2, NT = 256;
   constexpr int TK = ColShape<T, NX, NY>::TK;
-  __shared__ cx<T> buf[TK * padlen(NY)];
+  __shared__ cx<T> buf[TK * colpad(NY)];
   __shared__ cx<T> tw[NY];
   const int tid = threadIdx.x;
   load_twiddles(tw, twg, NY, tid, NT);
@@ -2611,17 +2611,17 @@ __global__ void __launch_bounds__(256) k_cols_small(const cx<T>* __restrict__ in
     const int c = i % TK, j = i / TK;
     cx<T> v = src[(size_t)j * MX + c];
     if (INV && j == NY / 2) v = cx<T>{0, 0};
-    buf[c * padlen(NY) + pidx(j)] = v;
+    buf[c * colpad(NY) + pidx(j)] = v;
   }
   __syncthreads();
-  fft_waves<T, NY, INV, TK, NT / 64>(buf, tw, tid);
+  fft_waves<T, NY, INV, TK, NT / 64, colpad(NY)>(buf, tw, tid);
   __syncthreads();
   cx<T>* dst = out + (size_t)L * NY * MX + k0;
   const T s = INV ? T(1) : T(1) / T(NY);
 #pragma unroll
   for (int i = tid; i < TK * NY; i += NT) {
     const int c = i % TK, j = i / TK;
-    cx<T> v = cscale(buf[c * padlen(NY) + pidx(j)], s);
+    cx<T> v = cscale(buf[c * colpad(NY) + pidx(j)], s);
     if (!INV && j == NY / 2) v = cx<T>{0, 0};
     dst[(size_t)j * MX + c] = v;
   }

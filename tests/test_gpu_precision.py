@@ -8,10 +8,12 @@
   L2; mean profile 1.2e-8, resolved TKE 3.3e-6, calc_spectra x-y spectra
   2.4e-5 at every wavenumber).
 * The FP32 + learned-closure path after 200 steps, against the FP64
-  TKE-closure run.  Asserted: the mean profile within 1 % (measured 0.24 %)
-  and the plane- and level-summed resolved TKE within 5 % (measured 1.7 %).
-  The per-wavenumber spectra are printed, not asserted.  They differ by up
-  to 42 % at k <= 8 and 87 % overall, so the 5 % spectra target is NOT met.
+  TKE-closure run.  Asserted: the mean profile within 1 % (measured 0.24 %),
+  the plane- and level-summed resolved TKE within 5 % (measured 1.7 %), and
+  the x-y spectra at three levels within 120 % per ring and 60 % up to k = 8
+  (measured 87 % and 42 %): bounds that a closure with no stress, or with
+  stresses off by a factor of a few, fails.  The 5 % spectra target against
+  the TKE closure is NOT met.
   The learned closure dissipates about 2.4x less than the TKE closure (its
   a-priori mean SGS dissipation is 0.010 against the TKE's 0.024 and the
   filtered fine run's 0.012).  The seeded noise therefore evolves differently
@@ -110,6 +112,9 @@ def test_learned_path_against_fp64_tke_256():
     e = _report("fp32+learned vs fp64 TKE after 200 steps:", sl, s64, ml, m64, 128)
     assert e["uxym"] < 1e-2, e                     # mean profile within 1 %
     assert abs(e["englez_ratio"] - 1.0) < 5e-2, e  # total resolved TKE within 5 %
+    # spectra: per ring within 120 % and within 60 % up to k = 8 (measured 87 %
+    # and 42 %); the 5 % target is not met (see the module docstring)
+    assert e["spectra"] < 1.2 and e["spectra_k<=8"] < 0.6, e
 
 
 def test_learned_small_grid_against_fp64_tke_256():
@@ -120,6 +125,7 @@ def test_learned_small_grid_against_fp64_tke_256():
     e = _report("fp32+learned (small grid) vs fp64 TKE after 200 steps:", sl, s64, ml, m64, 128)
     assert e["uxym"] < 1e-2, e
     assert abs(e["englez_ratio"] - 1.0) < 5e-2, e
+    assert e["spectra"] < 1.2 and e["spectra_k<=8"] < 0.6, e   # measured 82 % and 37 %
 
 
 def test_fp32_path_after_2000_steps_256():
