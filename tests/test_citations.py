@@ -10,7 +10,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = "/root/reference"
 CITE = re.compile(r"\b([A-Za-z0-9_]+\.(?:f90|F90)):(\d+(?:-\d+)?(?:,\s?\d+(?:-\d+)?)*)")
-# a bare continuation such as "`:26-27`" or ", :59-63" cites the file named last on the line
+# a bare continuation such as "`:26-27`" or ", :59-63" cites the file named before it on the line
 BARE = re.compile(r"(?<=[\s(`]):(\d+(?:-\d+)?(?:,\s?\d+(?:-\d+)?)*)")
 # files the driver writes, and the survey: not this repository's citations
 SKIP = {"SURVEY.md", "VERDICT.md", "ADVICE.md", "BASELINE.md", "BASELINE.json"}
@@ -26,13 +26,13 @@ def citations():
     for f in tracked():
         with open(os.path.join(ROOT, f), errors="replace") as fh:
             for lineno, line in enumerate(fh, 1):
-                last = None
+                named = [(m.start(), m.group(1)) for m in CITE.finditer(line)]
                 for m in CITE.finditer(line):
-                    last = m.group(1)
                     yield f, lineno, m.group(1), m.group(2)
-                if last:
-                    for m in BARE.finditer(line):
-                        yield f, lineno, last, m.group(1)
+                for m in BARE.finditer(line):
+                    before = [name for pos, name in named if pos < m.start()]
+                    if before:                       # the nearest file named before it
+                        yield f, lineno, before[-1], m.group(1)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference sources absent")
