@@ -12,12 +12,15 @@ import cf_oracle as O
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("field", ["e", "u"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_nonfinite_tke_is_reported(precision):
+def test_nonfinite_tke_is_reported(precision, field):
+    """A NaN in e is caught by the e update of the same stage; a NaN in u
+    reaches e in the same stage through the TKE advection."""
     import channelflow as C
     pr = O.Params(32, 32, 16, dt=0.01)
     u, v, w, e = O.initial_fields(pr)
-    e[8, 5, 7] = np.nan                          # one interior face, away from both walls
+    (e if field == "e" else u)[8, 5, 7] = np.nan   # one interior level, away from both walls
     g = C.ChannelFlow(C.RunParams(32, 32, 16, z0=pr.z0, dt=pr.dt, precision=precision))
     try:
         g.set_state(u, v, w, e)
