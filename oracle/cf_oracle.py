@@ -121,27 +121,45 @@ def _waves(n, length, ncut):
 # ----------------------------------------------------------------------------
 # Spectral layer (filter_fftw.f90)
 # ----------------------------------------------------------------------------
+# The DFTs go through numpy's pocketfft.  FFT_WORKERS > 1 hands them to
+# scipy.fft's multi-threaded pocketfft instead (the same DFT); only the CPU
+# baseline timing sets it.
+FFT_WORKERS = 1
+
+
+class _Fft:
+    def __getattr__(self, name):
+        if FFT_WORKERS > 1:
+            import functools
+            import scipy.fft
+            return functools.partial(getattr(scipy.fft, name), workers=FFT_WORKERS)
+        return getattr(np.fft, name)
+
+
+_fft = _Fft()
+
+
 def ffthrz_fwd(a):
     """ffthrz(fa,n1,n2,-1) (filter_fftw.f90:107-130): 2-D r2c scaled by 1/(n1 n2).
     Returns the (n2, n1/2+1) complex half spectrum."""
-    return np.fft.rfft2(a, norm="forward")
+    return _fft.rfft2(a, norm="forward")
 
 
 def ffthrz_inv(c, n1):
     """ffthrz(fa,n1,n2,+1): unscaled 2-D c2r (y inverse first, then x c2r)."""
-    return np.fft.irfft2(c, s=(c.shape[-2], n1), norm="forward")
+    return _fft.irfft2(c, s=(c.shape[-2], n1), norm="forward")
 
 
 def xderiv(a, length):
     """xderiv (filter_fftw.f90:138-187): 1-D spectral d/dx along rows, k=0 and
     the Nyquist coefficient zeroed, /n after the unscaled inverse."""
     n = a.shape[-1]
-    c = np.fft.rfft(a, axis=-1)
+    c = _fft.rfft(a, axis=-1)
     k = np.arange(n // 2 + 1) * 2.0 * math.pi / length
     c = c * (1j * k)
     c[..., 0] = 0.0
     c[..., n // 2] = 0.0
-    return np.fft.irfft(c, n=n, axis=-1)
+    return _fft.irfft(c, n=n, axis=-1)
 
 
 def yderiv(a, length):
@@ -149,12 +167,12 @@ def yderiv(a, length):
     n = a.shape[-2]
     if n == 1:
         return np.zeros_like(a)
-    c = np.fft.rfft(a, axis=-2)
+    c = _fft.rfft(a, axis=-2)
     k = np.arange(n // 2 + 1) * 2.0 * math.pi / length
     c = c * (1j * k)[:, None]
     c[..., 0, :] = 0.0
     c[..., n // 2, :] = 0.0
-    return np.fft.irfft(c, n=n, axis=-2)
+    return _fft.irfft(c, n=n, axis=-2)
 
 
 def expand(a, nxb, nyb):

@@ -23,10 +23,10 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 
 
 def test_cpu_baseline_record_at_a_small_size():
-    a = types.SimpleNamespace(cpu_sample_nz=8, nz=8, nx=32, re_tau=395.0)
+    a = types.SimpleNamespace(cpu_sample_nz=8, nz=8, nx=32, re_tau=395.0, cpu_cores=2)
     cb = bench.cpu_baseline(a)
     assert set(cb) == {"value", "unit", "cores", "kind", "sample"}
-    assert cb["kind"] == "port" and cb["cores"] == 1 and cb["value"] > 0
+    assert cb["kind"] == "port" and cb["cores"] == 2 and cb["value"] > 0
     assert cb["unit"] == "grid-pt RK3 steps/s" and "32x32x8" in cb["sample"]
 
 
