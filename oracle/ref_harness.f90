@@ -8,6 +8,8 @@
 !         sufsim, sufsim_upper) for iz = 1..nnz and writes the stresses and
 !         eddy diffusivities it leaves in the rings.
 ! mode 2: solves the tridiagonal systems of the input with the reference tridv.
+! mode 3: loads u, v (levels 1..nnz+1), w (faces 0..nnz), e (0..nnz+1) and p
+!         (1..nnz+1), calls the reference compmn(1, nnz) and writes meanbuf.
 !
 ! Stream files, native endianness: input on unit 10 (argv 1), output unit 11
 ! (argv 2).
@@ -27,6 +29,8 @@ program ref_harness
      call run_sgs()
   else if (mode == 2) then
      call run_tridv()
+  else if (mode == 3) then
+     call run_means()
   end if
   close(10)
   close(11)
@@ -134,5 +138,30 @@ contains
     call tridv(bdar, ny, nz)
     write(11) bdar(:,:,4)
   end subroutine run_tridv
+
+  subroutine run_means()
+    integer :: np(4), nptot(4), nm(4), nmtot(4)
+    read(10) nnx, nny, nnz
+    read(10) xl, yl, zl
+    isbgr = .true.
+    spectav = .true.
+    nnxb = 3*nnx/2
+    nnyb = 3*nny/2
+    xymb = 1.0/float(nnxb)/float(nnyb)
+    ncx = nnx/2+1
+    ncy = nny/2+1
+    nxy = nnx*nny
+    xym = 1./float(nnx*nny)
+    call setcombuf(np, nptot, nm, nmtot)
+    call allocate_arrays(1, nnz, np, nm)
+    u = 0.0
+    v = 0.0
+    w = 0.0
+    e = 0.0
+    p = 0.0
+    read(10) u(:,:,1:nnz+1), v(:,:,1:nnz+1), w(:,:,0:nnz), e(:,:,0:nnz+1), p(1:nnx,:,1:nnz+1)
+    call compmn(1, nnz)
+    write(11) meanbuf(:,1:nnz)
+  end subroutine run_means
 
 end program ref_harness

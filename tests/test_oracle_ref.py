@@ -1,6 +1,6 @@
 """Pins the oracle's SGS/wall-model layer and tridiagonal solver against the
 reference's own Fortran (complen.f90, compsgs1.f90, lowerbc.f90, upperbc.f90,
-tridv.f90, compiled in FP64 from the reference's sources by oracle/build_ref.sh
+tridv.f90, compmn.F90, compiled in FP64 from the reference's sources by oracle/build_ref.sh
 into oracle/_ref).  The reference's FFT-dependent files need FFTW 2.x, which is not
 in this image, so only this FFT-free subset is built.  Skipped when neither the
 binary nor the reference sources are present."""
@@ -115,3 +115,35 @@ def test_tridv_matches_reference(harness, tmp_path):
     out = run(harness, [np.array([2, ny, nz], dtype=np.int32), bdar], tmp_path).reshape(nz, ny)
     mine = O.tridv(b, d, a, r)
     assert np.abs(out - mine).max() < 1e-14 * np.abs(mine).max()
+
+
+def test_compmn_matches_reference(harness, tmp_path):
+    """Horizontal means and resolved moments against the reference compmn
+    (compmn.F90:1-64) on random fields, the top level's u/v (nnz+1) included.
+    The reference also forms a pressure fluctuation it never stores (pple
+    stays 0, allocate.f90:88,104); divz and the SGS rows are filled by comp1
+    and compsgs3, not compmn, so both sides hold zeros there.  Catches an
+    unshifted level k+1 mean in ul/vl, w averaged to the cell in wwle or
+    uwle, e taken one level low, and a dropped 1/2 in englez."""
+    pr = O.Params(16, 12, 10, dt=0.01)
+    rng = np.random.default_rng(23)
+    st = O.zeros_state(pr)
+    n, OF = pr.nnz, O.OF
+    shp = lambda m: (m, pr.nny, pr.nnx)       # noqa: E731
+    st.u[1 + OF: n + 2 + OF] = 1.0 + rng.standard_normal(shp(n + 1))
+    st.v[1 + OF: n + 2 + OF] = -0.3 + rng.standard_normal(shp(n + 1))
+    st.w[0 + OF: n + 1 + OF] = 0.1 + rng.standard_normal(shp(n + 1))
+    st.e[0 + OF: n + 2 + OF] = rng.uniform(0.1, 1.0, shp(n + 2))
+    p = rng.standard_normal(shp(n + 1))
+    st.divz = np.zeros(n)
+    st.sgs = {}
+    payload = [np.array([3, pr.nnx, pr.nny, pr.nnz], dtype=np.int32),
+               np.array([pr.xl, pr.yl, pr.zl], dtype=np.float64),
+               st.u[1 + OF: n + 2 + OF], st.v[1 + OF: n + 2 + OF], st.w[0 + OF: n + 1 + OF],
+               st.e[0 + OF: n + 2 + OF], p]
+    out = run(harness, payload, tmp_path).reshape(n, 19).T     # Fortran meanbuf(19, nnz)
+    mine = O.compmn(st, pr)
+    for i, name in enumerate(O.MEAN_NAMES):
+        scale = max(np.abs(out[i]).max(), 1e-30)
+        assert np.abs(mine[i] - out[i]).max() <= 1e-13 * scale, (name, mine[i], out[i])
+    assert np.any(out[[0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11]] != 0)
