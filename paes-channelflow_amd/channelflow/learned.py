@@ -17,7 +17,19 @@ The samples come from this code's own finer-grid run:
    <u_i u_j> - <u_i><u_j> of the cell-centred fine velocities, plus the box
    mean of the fine run's own TKE-closure stresses.
 5. The network is trained on the device by the library's BF16 MFMA forward,
-   backward and Adam kernels (SgsNet.train).
+   backward and Adam kernels (SgsNet.train).  Several candidates are trained
+   on the same samples from different initial weights and batch orders.
+6. Each candidate is run a posteriori: the LES of the validation case (the
+   learned-closure grid, FP32, from a seeded state whose seed differs from
+   the one the tests use), once with the network on the nnx x nny grid and
+   once on the 3/2 grid, against the FP64 TKE-closure run of the same state.
+   Candidates with the same a-priori error differ widely here, so the one
+   with the smallest validation score is kept: over both closure grids, the
+   larger of its mean-profile error over 1 % and its resolved-TKE deviation
+   over 5 %.  Only candidates whose stresses are
+   accepted at a fraction --min-accept of the cells or more compete (one that
+   falls back to the TKE closure nearly everywhere would match that closure
+   trivially); if none is, all do.
 
 ``evaluate`` measures a trained network a priori on held-out samples.  It
 compares the deviatoric stress error against three baselines:
@@ -39,7 +51,8 @@ import time
 
 import numpy as np
 
-from . import (CF_PREC_FP32, ChannelFlow, RunParams, SgsNet, z0_of_retau)
+from . import (CF_CLOSURE_GRID_DEALIASED, CF_CLOSURE_GRID_SMALL, CF_PREC_FP32, CF_PREC_FP64, ChannelFlow, RunParams, SgsNet,
+               z0_of_retau)
 
 FEATURE_NAMES = ("ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz", "delta", "wall")
 STRESS_NAMES = ("t11", "t22", "t33", "t12", "t13", "t23")
@@ -156,6 +169,75 @@ def _log(*a):
     print(*a, flush=True)
 
 
+_ENGLEZ, _UXYM = 0, 3                 # meanbuf rows (allocate.f90:89,93)
+
+
+def _validation_les(args, precision):
+    nx, nz = args.val_grid
+    z0 = z0_of_retau(args.re_tau)
+    dt = args.cfl / (3.2 * (3 * nx / 2) / (2 * math.pi) + 0.5 / (2.0 / nz))
+    g = ChannelFlow(RunParams(nx, nx, nz, z0=z0, dt=dt, precision=precision))
+    g.seeded_init(args.val_seed, 0.1)
+    return g
+
+
+def _means_after(g, steps):
+    """compmn means of the last of ``steps`` RK3 steps."""
+    g.step(steps - 1)
+    for s in (1, 2, 3):
+        g.stage(s, mnout=True)
+    g.sync()
+    return g.compmn()
+
+
+def select_candidate(vals, min_accept):
+    """Index of the candidate to keep: the lowest validation score among the
+    finite-score candidates accepted at a fraction min_accept of the cells or
+    more, else among all finite-score ones, else among all."""
+    fin = [i for i, v in enumerate(vals) if math.isfinite(v["score"])]
+    pool = [i for i in fin if vals[i]["accepted"] >= min_accept] or fin or list(range(len(vals)))
+    return min(pool, key=lambda i: vals[i]["score"])
+
+
+def validation_reference(args):
+    """Means of the FP64 TKE-closure run of the validation case."""
+    g = _validation_les(args, CF_PREC_FP64)
+    try:
+        return _means_after(g, args.val_steps)
+    finally:
+        g.close()
+
+
+def _validate_on(net, args, ref, grid):
+    g = _validation_les(args, CF_PREC_FP32)
+    try:
+        g.set_closure(net)
+        g.set_closure_grid(grid)
+        mb = _means_after(g, args.val_steps)
+        acc, tot = g.closure_counts()
+    finally:
+        g.close()
+    uxym = float(np.max(np.abs(mb[_UXYM] - ref[_UXYM]) / np.abs(ref[_UXYM])))
+    ratio = float(np.sum(mb[_ENGLEZ]) / np.sum(ref[_ENGLEZ]))
+    finite = bool(np.all(np.isfinite(mb)))
+    score = max(uxym / 0.01, abs(ratio - 1.0) / 0.05) if finite else float("inf")
+    return {"uxym": uxym, "englez_ratio": ratio, "accepted": acc / max(tot, 1), "score": score}
+
+
+def validate(net, args, ref):
+    """A-posteriori error of ``net`` on the validation case against the FP64
+    TKE-closure means ``ref``, per closure grid: the largest relative error
+    of the mean profile uxym, the ratio of the summed resolved TKE englez,
+    the accepted fraction and the score max(uxym / 0.01, |ratio - 1| / 0.05).
+    The candidate's score is the larger of the two grids' scores, its
+    accepted fraction the smaller."""
+    out = {"small": _validate_on(net, args, ref, CF_CLOSURE_GRID_SMALL),
+           "dealiased": _validate_on(net, args, ref, CF_CLOSURE_GRID_DEALIASED)}
+    out["score"] = max(out["small"]["score"], out["dealiased"]["score"])
+    out["accepted"] = min(out["small"]["accepted"], out["dealiased"]["accepted"])
+    return out
+
+
 def run(args):
     os.makedirs(args.out_dir, exist_ok=True)
     nx, nz = args.nx, args.nz
@@ -203,7 +285,6 @@ def run(args):
     net = SgsNet(0)
     net.set_norm(in_mu, in_sd, out_mu, out_sd)
     rng = np.random.default_rng(args.seed)
-    net.set_params(_init_params(rng))
     net.reserve(ntrain)
     off = 0
     for f, t in zip(chunks_f, chunks_t):
@@ -211,20 +292,30 @@ def run(args):
         off += f.shape[0]
     del chunks_f, chunks_t, chunks_e
     _log(f"[learned] {ntrain} training samples on the device, {nheld} held out")
-    hist = []
-    step = 0
-    for (nsteps, lr) in args.schedule:
-        for k in range(0, nsteps, 500):
-            m = min(500, nsteps - k)
-            losses = net.train(m, args.batch, lr, seed=args.seed + step)
-            step += m
-            hist.append((step, float(losses[-50:].mean())))
-            _log(f"[learned] train step {step}, lr {lr:g}, batch loss {hist[-1][1]:.4f}, "
-                 f"{time.time() - t0:.0f} s")
     f, t, e = held
-    pred = net.forward(f)
-    ev = evaluate(pred, f, t, e)
-    _log("[learned] held-out a-priori:", json.dumps(ev))
+    ref = validation_reference(args)
+    cands = []
+    for c in range(args.candidates):
+        net.set_params(_init_params(np.random.default_rng(args.seed + c)))
+        net.reset_optimizer()
+        hist = []
+        step = 0
+        for (nsteps, lr) in args.schedule:
+            for k in range(0, nsteps, 500):
+                m = min(500, nsteps - k)
+                losses = net.train(m, args.batch, lr, seed=args.seed + 1000003 * c + step)
+                step += m
+                hist.append((step, float(losses[-50:].mean())))
+        ev = evaluate(net.forward(f), f, t, e)
+        val = validate(net, args, ref)
+        cands.append({"params": net.get_params(), "loss_history": hist, "final_loss": hist[-1][1],
+                      "held_out": ev, "validation": val})
+        _log(f"[learned] candidate {c}: batch loss {hist[-1][1]:.4f}, held-out MSE "
+             f"{ev['mse_dev']['mlp']:.3e}, validation {json.dumps(val)}, {time.time() - t0:.0f} s")
+    best = select_candidate([cd["validation"] for cd in cands], args.min_accept)
+    net.set_params(cands[best]["params"])
+    hist, ev = cands[best]["loss_history"], cands[best]["held_out"]
+    _log(f"[learned] kept candidate {best}; held-out a-priori:", json.dumps(ev))
     wpath = os.path.join(args.out_dir, "sgs_mlp.cfmlp")
     net.save(wpath)
     meta = {
@@ -232,6 +323,12 @@ def run(args):
         "seed": args.seed, "spinup_steps": args.spinup, "snapshots": args.snaps, "every": args.every,
         "filters": filters, "train_samples": ntrain, "held_out_samples": nheld, "batch": args.batch,
         "schedule": args.schedule, "loss_history": hist, "held_out": ev, "run": state_stats,
+        "validation_case": {"grid": [args.val_grid[0], args.val_grid[0], args.val_grid[1]],
+                            "seed": args.val_seed, "steps": args.val_steps,
+                            "reference": "FP64 TKE closure", "closure_grids": ["nnx x nny", "3/2"]},
+        "min_accept": args.min_accept,
+        "candidates": [{k: cd[k] for k in ("final_loss", "held_out", "validation")} for cd in cands],
+        "selected": best,
         "wall_s": time.time() - t0,
     }
     with open(os.path.join(args.out_dir, "sgs_mlp.json"), "w") as fh:
@@ -271,6 +368,13 @@ def main(argv=None):
                     default=[(4000, 1e-3), (3000, 3e-4), (3000, 1e-4)],
                     help="steps:lr,steps:lr,...")
     ap.add_argument("--fixture", type=int, default=16384)
+    ap.add_argument("--candidates", type=int, default=8, help="networks trained on the same samples")
+    ap.add_argument("--val-grid", type=lambda s: tuple(int(x) for x in s.split(",")), default=(256, 128),
+                    help="nnx(=nny),nnz of the a-posteriori validation case")
+    ap.add_argument("--val-seed", type=int, default=4242, help="seeded state of the validation case")
+    ap.add_argument("--val-steps", type=int, default=200)
+    ap.add_argument("--min-accept", type=float, default=0.3,
+                    help="least accepted-cell fraction on the validation case for a candidate to be kept")
     ap.add_argument("--out-dir", required=True)
     run(ap.parse_args(argv))
 

