@@ -84,6 +84,8 @@ BASIS = {
                   "the network's work is on-chip)",
     "k_sgs_rows": "13 x N x 4 B per launch (6 small x-spectral rows and the physical e in, "
                   "6 divergence rows out; the network's work is on-chip)",
+    "k_rows_lean": "19.5 x N x 4 B per launch (the u, v, w, e, ey 3/2-grid rows in, the 8 flux "
+                   "rows out, once each)",
 }
 
 # HBM bytes per dispatch from the FETCH_SIZE and WRITE_SIZE counters (KiB per
@@ -98,6 +100,7 @@ MEASURED_TRAFFIC = {
     ("k_rows_mlp", 256, 128, "fp32"): round((2 * 202212.8 + 392256.0) * 1024),
     # profiles/learned_small_grid_counters_256x256x128.md
     ("k_sgs_rows", 256, 128, "fp32"): round((2 * 117387.6 + 196736.0) * 1024),
+    ("k_rows_lean", 256, 128, "fp32"): round((2 * 125529.2 + 391680.0) * 1024),
 }
 
 # SQ_INSTS_VALU per dispatch (wavefront instructions) at 256x256x128 FP32, from
@@ -107,12 +110,14 @@ MEASURED_VALU = {
     ("k_rows", 256, 128, "fp32"): 227.93e6,
     ("k_rows_mlp", 256, 128, "fp32"): 738.28e6,
     ("k_sgs_rows", 256, 128, "fp32"): 294.34e6,
+    ("k_rows_lean", 256, 128, "fp32"): 174.61e6,
 }
 # SQ_ACTIVE_INST_VALU per dispatch (4-cycle issue slots the waves spent in
 # VALU instructions; a transcendental takes two) where profiled
-# (profiles/sq_counters_learned_256x256x128.md).
+# (profiles/learned_small_grid_counters_256x256x128.md).
 MEASURED_VALU_ACTIVE = {
     ("k_sgs_rows", 256, 128, "fp32"): 346.64e6,
+    ("k_rows_lean", 256, 128, "fp32"): 175.78e6,
 }
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
@@ -292,8 +297,11 @@ def main():
     per_kernel_gbs = {k: kernel_bytes(k, pts, 4, lean=lean) / (t * 1e-3) / 1e9
                       for k, t in kern.items() if kernel_bytes(k, pts, 4, lean=lean) and t > 0}
     # on the 3/2 grid the learned path runs the row kernel with the network
-    # inside (k_rows_mlp): same row inputs and outputs, same algorithmic bytes
-    dom_label = "k_rows_mlp" if dom == "k_rows" and args.closure_grid == "dealiased" else dom
+    # inside (k_rows_mlp): same row inputs and outputs, same algorithmic bytes;
+    # on the nnx x nny grid it runs the lean row sweep (k_rows<T, NX, true>)
+    dom_label = dom
+    if dom == "k_rows":
+        dom_label = "k_rows_lean" if lean else "k_rows_mlp"
     traffic = MEASURED_TRAFFIC.get((dom_label, args.nx, args.nz, "fp32"))
     line = {
         "metric": "grid-pt RK3 steps/s @256^2x128: FP32+learned SGS vs FP64 plain; % HBM peak",
