@@ -41,7 +41,8 @@ def test_algorithmic_bytes_of_the_stage_kernels():
     # the counter-measured HBM traffic of the profiled kernels is at least
     # their algorithmic bytes and within 6 % of them
     for (k, nx, nz, _), traffic in bench.MEASURED_TRAFFIC.items():
-        alg = bench.kernel_bytes("k_rows" if k == "k_rows_mlp" else k, nx * nx * nz, 4)
+        name = "k_rows" if k in ("k_rows_mlp", "k_rows_lean") else k
+        alg = bench.kernel_bytes(name, nx * nx * nz, 4, lean=k == "k_rows_lean")
         assert alg <= traffic <= 1.06 * alg, (k, traffic, alg)
 
 
