@@ -376,7 +376,10 @@ def main(argv=None):
     ap.add_argument("--min-accept", type=float, default=0.3,
                     help="least accepted-cell fraction on the validation case for a candidate to be kept")
     ap.add_argument("--out-dir", required=True)
-    run(ap.parse_args(argv))
+    args = ap.parse_args(argv)
+    if args.candidates < 1:
+        ap.error("--candidates must be at least 1")
+    run(args)
 
 
 if __name__ == "__main__":

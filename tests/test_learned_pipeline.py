@@ -2,12 +2,14 @@
 features of channelflow.learned use the same stencils as the closure in the
 LES.  That is checked against the oracle's 3/2-grid cell features at the grid
 points the two grids share, for a band-limited state.  Also checked: the
-subfilter-stress targets, and the a-priori metrics on constructed cases."""
+subfilter-stress targets, the a-priori metrics on constructed cases, and the
+rule that picks a candidate network by its a-posteriori validation score."""
 import math
 import sys
 import os
 
 import numpy as np
+import pytest
 
 import cf_oracle as O
 import sgs_mlp as M
@@ -92,3 +94,9 @@ def test_candidate_selection_rule():
     assert L.select_candidate(vals, 0.0) == 1
     assert L.select_candidate([v(0.1, 3.0), v(0.2, 1.5), v(0.05, 0.5)], 0.3) == 2
     assert L.select_candidate([v(0.4, float("inf")), v(0.1, 9.0)], 0.3) == 1
+
+
+def test_pipeline_refuses_zero_candidates():
+    L = _learned()
+    with pytest.raises(SystemExit):
+        L.main(["--candidates", "0", "--out-dir", "unused"])
