@@ -1159,6 +1159,10 @@ int cf_calc_spectra_noalloc(cf_ctx* ctx, int32_t nlev, const int32_t* zlevels, d
     return CF_E_HIP;
   }
   const int nb = ctx->nx / 2 + 1;
+  if (nb > kSpecMaxBins) {
+    ctx->err = "cf_calc_spectra_noalloc's scratch holds nnx <= " + std::to_string(2 * (kSpecMaxBins - 1));
+    return CF_E_ARG;
+  }
   double* ring = base;
   double* outp = base + kSpecRing;
   int32_t* dz = (int32_t*)(base + kSpecRing + kSpecOut);
