@@ -2398,6 +2398,261 @@ __global__ void __launch_bounds__(64) k_press(PressArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_press_part: the arithmetic of k_press with each mode's z-column split into
+// chunks of C levels, one thread per (mode, chunk), the chunk held in
+// registers: every input is read once and every output written once (no
+// eliminated-rhs scratch, no second read of U, V, W and Rn, no pivot table).
+//
+// The tridiagonal solve (press.F90:44-76, tridv.f90:34-53) is the partition
+// (spike) method.  The last level of every chunk but the last is an
+// interface; a chunk's interior levels solve A_c p = r_c - I_l e_first -
+// I_r e_last, so p = y - I_l g - I_r h with A_c y = r_c, A_c g = e_first,
+// A_c h = e_last.  The row of interface c then couples I_{c-1}, I_c, I_{c+1}:
+//   -g_c(last) I_{c-1} + (b - h_c(last) - g_{c+1}(first)) I_c
+//     - h_{c+1}(first) I_{c+1} = r_c(interface) - y_c(last) - y_{c+1}(first),
+// a diagonally dominant tridiagonal system of nc-1 rows that one thread per
+// mode solves in LDS by elimination.  Each chunk then back-substitutes its
+// interior with the interface values folded into the eliminated rhs.  The
+// solution differs from the serial sweep's by rounding: the interface system
+// inherits the diagonal dominance of -dz^2 k^2 - 2 (k^2 >= 0.05 for every mode
+// but (0,0), checked on the host).
+//
+// grid: ceil(modes / 16) + 1 workgroups of 16 NP threads, thread = chunk * 16
+// + mode (a wave's loads are four 16-mode runs of one level); the last
+// workgroup takes mode (0,0).  Needs nc = ceil(nz / C) <= NP.
+// ---------------------------------------------------------------------------
+template <typename T, int C, int NP, int MW>
+__global__ void __launch_bounds__(MW * NP) __attribute__((amdgpu_waves_per_eu(2)))
+k_press_part(PressArgs<T> a) {
+  if (blockIdx.x == gridDim.x - 1) {
+    press_zero_mode(a);
+    return;
+  }
+  // per chunk and mode: y at the first and last interior level (complex), g
+  // and h there, the interface row's rhs (complex)
+  enum { YF = 0, YL = 2, GF = 4, GL = 5, HF = 6, HL = 7, RI = 8, NQ = 10 };
+  __shared__ double post[NP][NQ][MW];
+  __shared__ double beta[NP][MW];       // reciprocal pivots of the interface system
+  __shared__ cx<double> itf[NP][MW];
+  const int tid = threadIdx.x, mi = tid % MW, c = tid / MW;
+  const int modes = a.mx * a.ny;
+  const int m = blockIdx.x * MW + mi;
+  const int j = m / a.mx, kx = m - j * a.mx;
+  const bool act = m < modes && m != 0 && j != a.ny / 2;   // (0,0) and the Nyquist row excluded
+  const int nz = a.nz;
+  const int nc = (nz + C - 1) / C;
+  const int lo = c * C + 1;
+  const int n = (lo + C - 1 <= nz ? lo + C - 1 : nz) - lo + 1;   // levels of this chunk
+  const bool live = act && c < nc;
+  const bool right = c < nc - 1;                                  // its last level is an interface
+  const int ni = right ? n - 1 : n;                               // interior levels
+  double kxp = 0.0, kyp = 0.0;
+  if (act) {
+    kxp = a.kxd[kx];
+    kyp = a.kyd[j];
+  }
+  const double dz2 = a.dz * a.dz;
+  const double k2 = kxp * kxp + kyp * kyp;       // xk2 (init.f90:33-42)
+  const size_t pl = (size_t)modes;
+  const bool sub = (a.istage != 1);
+
+  // loads: the chunk's levels, and w, rw of the level below it
+  cx<T> lu[C], lv[C], lw[C], ln0[C], ln1[C], ln2[C], ou[C], ov[C], ow[C];
+  const cx<T> zt = {T(0), T(0)};
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    lu[q] = lv[q] = lw[q] = ln0[q] = ln1[q] = ln2[q] = ou[q] = ov[q] = ow[q] = zt;
+    if (live && q < n) {
+      const size_t o = (size_t)(lo + q) * pl + m;
+      lu[q] = a.U[0][o]; lv[q] = a.U[1][o]; lw[q] = a.U[2][o];
+      ln0[q] = a.Rn[0][o]; ln1[q] = a.Rn[1][o]; ln2[q] = a.Rn[2][o];
+      if (sub) { ou[q] = a.Ro[0][o]; ov[q] = a.Ro[1][o]; ow[q] = a.Ro[2][o]; }
+    }
+  }
+  cx<T> wb = zt, nb = zt, ob = zt;
+  if (live && lo > 1) {
+    const size_t o = (size_t)(lo - 1) * pl + m;
+    wb = a.U[2][o];
+    nb = a.Rn[2][o];
+    if (sub) ob = a.Ro[2][o];
+  }
+
+  // substep, rhs and the interior elimination (comp1.f90:126-133,
+  // comprhs.f90:17-35, tridv.f90:34-43); g's rhs is e_first
+  cx<double> wm_prev = {0, 0}, rw_prev = {0, 0};
+  if (lo > 1) {
+    wm_prev = widen(wb);
+    if (sub) wm_prev = csub(wm_prev, cscale(widen(ob), a.dtab2));   // level lo-1 < nz
+    rw_prev = widen(nb);
+  }
+  cx<double> ry[C], rif = {0, 0};
+  double rg[C], dd[C];
+  double d_prev = 0.0, rg_prev = 0.0;
+  cx<double> ry_prev = {0, 0};
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    ry[q] = {0, 0};
+    rg[q] = 0.0;
+    dd[q] = 0.0;
+    if (live && q < n) {
+      const int z = lo + q;
+      cx<double> u = widen(lu[q]), v = widen(lv[q]), w = widen(lw[q]);
+      if (sub) {
+        u = csub(u, cscale(widen(ou[q]), a.dtab2));
+        v = csub(v, cscale(widen(ov[q]), a.dtab2));
+        if (z != nz) w = csub(w, cscale(widen(ow[q]), a.dtab2));
+        lu[q] = cx<T>{(T)u.x, (T)u.y};
+        lv[q] = cx<T>{(T)v.x, (T)v.y};
+        lw[q] = cx<T>{(T)w.x, (T)w.y};
+      }
+      const cx<double> rw = widen(ln2[q]);
+      const cx<double> fx = cadd(widen(ln0[q]), cscale(u, a.dtabm));
+      const cx<double> fy = cadd(widen(ln1[q]), cscale(v, a.dtabm));
+      cx<double> rhs = cadd(cmuli(cscale(fx, kxp)), cmuli(cscale(fy, kyp)));
+      const cx<double> dw = cadd(cscale(csub(w, wm_prev), a.dtabm), csub(rw, rw_prev));
+      rhs = cadd(rhs, cscale(dw, a.dzm));
+      wm_prev = w;
+      rw_prev = rw;
+      const cx<double> r = cscale(rhs, dz2);
+      if (q < ni) {
+        const double diag = -dz2 * k2 - ((z == 1 || z == nz) ? 1.0 : 2.0);
+        if (q == 0) {
+          dd[q] = 1.0 / diag;
+          ry[q] = r;
+          rg[q] = 1.0;
+        } else {
+          dd[q] = 1.0 / (diag - d_prev);         // sub- and super-diagonal are 1
+          ry[q] = csub(r, cscale(ry_prev, d_prev));
+          rg[q] = -d_prev * rg_prev;
+        }
+        d_prev = dd[q];
+        ry_prev = ry[q];
+        rg_prev = rg[q];
+      } else {
+        rif = r;                                 // q == ni: the interface level
+      }
+    }
+  }
+
+  // y, g, h at the interior's first and last level (tridv.f90:44-53)
+  {
+    cx<double> y = {0, 0}, yl = {0, 0};
+    double g = 0.0, h = 0.0, gl = 0.0, hl = 0.0;
+#pragma unroll
+    for (int q = C - 1; q >= 0; --q) {
+      if (live && q < ni) {
+        if (q == ni - 1) {
+          y = cscale(ry[q], dd[q]);
+          g = dd[q] * rg[q];
+          h = dd[q];
+          yl = y;
+          gl = g;
+          hl = h;
+        } else {
+          y = cscale(csub(ry[q], y), dd[q]);
+          g = dd[q] * (rg[q] - g);
+          h = -dd[q] * h;
+        }
+      }
+    }
+    if (c == 0) g = gl = 0.0;                    // no coupling below level 1
+    if (!right) h = hl = 0.0;                    // none above level nz
+    post[c][YF][mi] = y.x;
+    post[c][YF + 1][mi] = y.y;
+    post[c][YL][mi] = yl.x;
+    post[c][YL + 1][mi] = yl.y;
+    post[c][GF][mi] = g;
+    post[c][GL][mi] = gl;
+    post[c][HF][mi] = h;
+    post[c][HL][mi] = hl;
+    post[c][RI][mi] = rif.x;
+    post[c][RI + 1][mi] = rif.y;
+  }
+  __syncthreads();
+
+  // the interface system, one thread per mode (loops unrolled over the
+  // largest size so that the LDS reads of later rows are issued early)
+  if (c == 0 && act) {
+    const double bif = -dz2 * k2 - 2.0;           // interfaces lie strictly inside 1..nz
+    double rbprev = 0.0, supprev = 0.0;
+    cx<double> qprev = {0, 0};
+#pragma unroll
+    for (int ci = 0; ci < NP - 1; ++ci) {
+      if (ci < nc - 1) {
+        const double diag = bif - post[ci][HL][mi] - post[ci + 1][GF][mi];
+        cx<double> q = {post[ci][RI][mi] - post[ci][YL][mi] - post[ci + 1][YF][mi],
+                        post[ci][RI + 1][mi] - post[ci][YL + 1][mi] - post[ci + 1][YF + 1][mi]};
+        double bt = diag;
+        if (ci > 0) {
+          const double f = -post[ci][GL][mi] * rbprev;   // row ci's coefficient of I_{ci-1} / pivot
+          bt = diag - f * supprev;
+          q = csub(q, cscale(qprev, f));
+        }
+        const double rb = 1.0 / bt;
+        beta[ci][mi] = rb;
+        itf[ci][mi] = q;
+        rbprev = rb;
+        qprev = q;
+        supprev = -post[ci + 1][HF][mi];          // row ci's coefficient of I_{ci+1}
+      }
+    }
+    cx<double> inext = {0, 0};
+#pragma unroll
+    for (int ci = NP - 2; ci >= 0; --ci) {
+      if (ci < nc - 1) {
+        cx<double> q = itf[ci][mi];
+        if (ci < nc - 2) q = csub(q, cscale(inext, -post[ci + 1][HF][mi]));
+        inext = cscale(q, beta[ci][mi]);
+        itf[ci][mi] = inext;
+      }
+    }
+  }
+  __syncthreads();
+
+  // back-substitution fused with the projection and the RK update
+  // (tridv.f90:44-53, comp2.f90:17-40)
+  if (live) {
+    const cx<double> il = (c > 0) ? itf[c - 1][mi] : cx<double>{0, 0};
+    const cx<double> ir = right ? itf[c][mi] : cx<double>{0, 0};
+    cx<double> p_next = {0, 0};
+    if (right) {                                  // p at level lo + n, the next chunk's first
+      const cx<double> i2 = (c + 1 < nc - 1) ? itf[c + 1][mi] : cx<double>{0, 0};
+      const cx<double> yf1 = {post[c + 1][YF][mi], post[c + 1][YF + 1][mi]};
+      p_next = csub(csub(yf1, cscale(ir, post[c + 1][GF][mi])), cscale(i2, post[c + 1][HF][mi]));
+    }
+#pragma unroll
+    for (int q = C - 1; q >= 0; --q) {
+      if (q < n) {
+        const int z = lo + q;
+        cx<double> p;
+        if (q == ni) {
+          p = ir;                                 // the interface level
+        } else {
+          const cx<double> rp = csub(ry[q], cscale(il, rg[q]));
+          p = (q == ni - 1) ? cscale(csub(rp, ir), dd[q]) : cscale(csub(rp, p_next), dd[q]);
+        }
+        const cx<double> ru = csub(widen(ln0[q]), cmuli(cscale(p, kxp)));   // comp2.f90:18-19
+        const cx<double> rv = csub(widen(ln1[q]), cmuli(cscale(p, kyp)));   // comp2.f90:23-24
+        cx<double> rw = {0, 0}, wn = {0, 0};
+        if (z != nz) {
+          rw = csub(widen(ln2[q]), cscale(csub(p_next, p), a.dzm));        // comp2.f90:29
+          wn = cadd(widen(lw[q]), cscale(rw, a.dtab1));                     // comp2.f90:37
+        }
+        const size_t o = (size_t)z * pl + m;
+        st2(a.U[0] + o, cadd(widen(lu[q]), cscale(ru, a.dtab1)));          // comp2.f90:34-35
+        st2(a.U[1] + o, cadd(widen(lv[q]), cscale(rv, a.dtab1)));
+        st2(a.U[2] + o, wn);
+        st2(a.Ro[0] + o, ru);
+        st2(a.Ro[1] + o, rv);
+        st2(a.Ro[2] + o, rw);
+        a.P[o] = p;
+        p_next = p;
+      }
+    }
+  }
+}
+
 // Load-time means: the band (0,0) coefficients of u and v take the FP64
 // plane means M0 (rounded to T, imaginary part 0); one thread per level.
 template <typename T>

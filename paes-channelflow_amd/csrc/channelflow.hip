@@ -96,6 +96,18 @@ const char* kKernelNames[CF_NKERN] = {"k_wall",        "k_col_synth", "k_divz_ba
 
 struct Engine;
 
+// Levels per chunk of k_press_part: its registers hold 20 (FP32) or 32 (FP64)
+// VGPRs per level, within 256 at two waves per SIMD.
+template <typename T>
+constexpr int PRESS_CHUNK = sizeof(T) == 4 ? 8 : 4;
+// Modes per workgroup of k_press_part: runs of 256 bytes per level and field
+// in a wave's loads and stores (32 FP32 or 16 FP64 modes), and at most 512
+// threads, so that its two waves per SIMD fit 256 VGPRs each.
+template <typename T>
+constexpr int press_modes(int np) {
+  return (int)(256 / sizeof(cx<T>)) * np <= 512 ? (int)(256 / sizeof(cx<T>)) : 512 / np;
+}
+
 struct cf_ctx {
   cf_params p{};
   int nx = 0, ny = 0, nz = 0, mx = 0, nxb = 0, nyb = 0, slots = 0, zchunk = 16, zchunk_mlp = 32, zchunk_sgs = 32,
@@ -119,6 +131,7 @@ struct cf_ctx {
   cx<double>* tw64NX = nullptr;
   double* piv = nullptr;        // k_press reciprocal pivots [slots][modes] (set at create)
   cx<double>* sr = nullptr;     // k_press forward-eliminated rhs [slots][modes]
+  int press_np = 0;             // chunks per mode of k_press_part; 0: the serial k_press
   double* M0 = nullptr;         // FP64 (0,0) coefficients of u, v, ru, rv [4][slots]
   cx<double>* band64 = nullptr; // FP32 contexts: FP64 band of the state crossing the boundary
   double* part = nullptr;
@@ -174,6 +187,20 @@ int dalloc(cf_ctx* ctx, X** ptr, size_t count) {
   ctx->bytes += (int64_t)nb;
   *ptr = (X*)p;
   return CF_OK;
+}
+
+// Chunks per mode of k_press_part: the smallest of 4, 8, 16, 32 that holds
+// ceil(nz / PRESS_CHUNK) chunks.  0 selects the serial k_press: more levels
+// than 32 chunks hold, or a wavenumber so small (k^2 < 0.05) that the
+// interface system's diagonal dominance no longer bounds its rounding.
+template <typename T>
+int press_chunks(const cf_ctx* ctx) {
+  const int nc = (ctx->nz + PRESS_CHUNK<T> - 1) / PRESS_CHUNK<T>;
+  const double kx1 = 2.0 * PI / ctx->p.xl, ky1 = 2.0 * PI / ctx->p.yl;
+  if (std::min(kx1 * kx1, ky1 * ky1) < 0.05) return 0;
+  for (int np : {4, 8, 16, 32})
+    if (nc <= np) return np;
+  return 0;
 }
 
 template <typename T>
@@ -240,8 +267,11 @@ struct EngineT : Engine {
     if ((rc = dalloc(ctx, &ctx->P, band)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->xs64, band)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->tw64NX, (size_t)NX)) != CF_OK) return rc;
-    if ((rc = dalloc(ctx, &ctx->piv, S * NY * MX)) != CF_OK) return rc;
-    if ((rc = dalloc(ctx, &ctx->sr, S * NY * MX)) != CF_OK) return rc;
+    ctx->press_np = press_chunks<T>(ctx);
+    if (!ctx->press_np) {                                   // the serial sweep's tables
+      if ((rc = dalloc(ctx, &ctx->piv, S * NY * MX)) != CF_OK) return rc;
+      if ((rc = dalloc(ctx, &ctx->sr, S * NY * MX)) != CF_OK) return rc;
+    }
     if ((rc = dalloc(ctx, &ctx->M0, 4 * S)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->part, 7 * S * NYB)) != CF_OK) return rc;
     if ((rc = dalloc(ctx, &ctx->red, 7 * S)) != CF_OK) return rc;
@@ -273,10 +303,12 @@ struct EngineT : Engine {
     HIPCHK(hipMemcpy(ctx->kyp, kyt.data(), NY * sizeof(T), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->kxd, kx.data(), MX * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->kyd, ky.data(), NY * sizeof(double), hipMemcpyHostToDevice));
-    // the pressure solve's reciprocal pivots depend on k^2 and dz only
-    hipLaunchKernelGGL(cf::k_press_pivots, dim3((MX * NY + 63) / 64), dim3(64), 0, ctx->stream, ctx->piv,
-                       ctx->kxd, ctx->kyd, MX, NY, ctx->nz, ctx->p.zl / ctx->nz);
-    HIPCHK(hipGetLastError());
+    // the serial sweep's reciprocal pivots depend on k^2 and dz only
+    if (!ctx->press_np) {
+      hipLaunchKernelGGL(cf::k_press_pivots, dim3((MX * NY + 63) / 64), dim3(64), 0, ctx->stream, ctx->piv,
+                         ctx->kxd, ctx->kyd, MX, NY, ctx->nz, ctx->p.zl / ctx->nz);
+      HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CF_OK;
   }
@@ -509,8 +541,21 @@ struct EngineT : Engine {
       a.dtab2 = c.dtab2;
       a.dtabm = c.dtabm;
       const int modes = MX * NY;
-      // one lane per mode, and one more workgroup for mode (0,0)
-      hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 63) / 64 + 1), dim3(64), 0, s, a);
+      if (ctx->press_np) {
+        // 16 modes x press_np chunks per workgroup, and one more for mode (0,0)
+        constexpr int CH = PRESS_CHUNK<T>;
+        const int np = ctx->press_np, mw = press_modes<T>(np);
+        const dim3 g((modes + mw - 1) / mw + 1), b(mw * np);
+        switch (np) {
+          case 4: hipLaunchKernelGGL((cf::k_press_part<T, CH, 4, press_modes<T>(4)>), g, b, 0, s, a); break;
+          case 8: hipLaunchKernelGGL((cf::k_press_part<T, CH, 8, press_modes<T>(8)>), g, b, 0, s, a); break;
+          case 16: hipLaunchKernelGGL((cf::k_press_part<T, CH, 16, press_modes<T>(16)>), g, b, 0, s, a); break;
+          default: hipLaunchKernelGGL((cf::k_press_part<T, CH, 32, press_modes<T>(32)>), g, b, 0, s, a); break;
+        }
+      } else {
+        // one lane per mode, and one more workgroup for mode (0,0)
+        hipLaunchKernelGGL(cf::k_press<T>, dim3((modes + 63) / 64 + 1), dim3(64), 0, s, a);
+      }
     }
     mark(true);
     {

@@ -55,6 +55,57 @@ def test_fp64_stages_match_oracle(n, nz):
     g.close()
 
 
+@pytest.mark.parametrize("nz,xl,kernel", [
+    (9, 2 * np.pi, "partitioned: chunks of 4, 4 and 1 levels"),
+    (13, 2 * np.pi, "partitioned: chunks of 4, 4, 4 and 1 levels"),
+    (129, 2 * np.pi, "serial: 33 chunks of 4 levels exceed 32"),
+    (16, 32.0, "serial: k^2 = (2 pi / 32)^2 < 0.05"),
+])
+def test_fp64_pressure_kernels_match_oracle(nz, xl, kernel):
+    """The pressure solve, projection and update per mode: the partitioned
+    kernel (z-chunks of 4 levels on the FP64 path, a tridiagonal interface
+    system) at chunkings whose last chunk holds one level, and the serial
+    sweep where the library selects it (more than 32 chunks, or a domain long
+    enough that k^2 < 0.05), all against the oracle's serial tridv after a
+    full step."""
+    import channelflow as C
+    pr = O.Params(32, 32, nz, xl=xl, dt=0.004)
+    u, v, w, e = O.initial_fields(pr)
+    st = O.set_state(pr, u, v, w, e)
+    g = make(pr, C.CF_PREC_FP64)
+    g.set_state(u, v, w, e)
+    O.step(st, pr)
+    g.step(1)
+    gs = g.get_state()
+    ref = fields(st, pr)
+    errs = {k: rel(gs[k], ref[k]) for k in "uvwep"}
+    print(kernel, errs, flush=True)
+    for k in "uvwep":
+        assert errs[k] < 1e-10, (kernel, k, errs)
+    g.close()
+
+
+@pytest.mark.parametrize("nz", [17, 40])
+def test_fp32_pressure_chunkings_close_to_fp64(nz):
+    """FP32 path (z-chunks of 8 levels: 8, 8, 1 at nz = 17; 5 full chunks at
+    40) after one step against the FP64 path on the same state."""
+    import channelflow as C
+    pr = O.Params(64, 64, nz, dt=0.004)
+    u, v, w, e = O.initial_fields(pr)
+    outs = []
+    for prec in (C.CF_PREC_FP32, C.CF_PREC_FP64):
+        g = make(pr, prec)
+        g.set_state(u, v, w, e)
+        g.step(1)
+        outs.append(g.get_state())
+        g.close()
+    errs = {k: rel(outs[0][k], outs[1][k]) for k in "uvwep"}
+    print(nz, errs, flush=True)
+    for k in "uvwe":
+        assert errs[k] < 5e-5, (k, errs)
+    assert errs["p"] < 5e-3, errs
+
+
 def test_fp64_tendencies_match_oracle():
     import channelflow as C
     pr = O.Params(32, 32, 32, dt=0.01)
