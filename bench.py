@@ -348,6 +348,7 @@ def main():
             "basis": "436 compulsory HBM bytes per grid-point RK3 step x grid-pt steps/s",
         },
         "sustained": r32["sustained"],
+        "device_bytes": r32["device_bytes"],
         "kernel_ms_per_stage": kern,
         "kernel_gbs": per_kernel_gbs,
         "fp64_plain": None,
@@ -357,7 +358,8 @@ def main():
     if rt is not None:
         vt = pts * args.steps / (rt["ms"] * 1e-3)
         line["fp32_tke"] = {"value": vt, "ms_per_step": rt["ms"] / args.steps,
-                            "kernel_ms_per_stage": rt["per_kernel"], "finite": rt["finite"]}
+                            "kernel_ms_per_stage": rt["per_kernel"], "finite": rt["finite"],
+                            "device_bytes": rt["device_bytes"]}
     if ro is not None:
         line["learned_other_grid"] = {
             "closure_grid": other, "value": pts * args.steps / (ro["ms"] * 1e-3),
@@ -369,7 +371,7 @@ def main():
             "value": v64, "ms_per_step": r64["ms"] / args.steps,
             "hbm_frac": BYTES_PER_GPSTEP["fp64"] * v64 / 1e9 / HBM_PEAK_GBS,
             "learned_fp32_over_fp64": value / v64, "kernel_ms_per_stage": r64["per_kernel"],
-            "finite": r64["finite"],
+            "finite": r64["finite"], "device_bytes": r64["device_bytes"],
         }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (numpy oracle, one step)", file=sys.stderr, flush=True)
