@@ -17,6 +17,7 @@
 // computed on the host in double precision and staged in LDS by the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace cf {
 
@@ -186,11 +187,56 @@ __device__ __forceinline__ void fft_sync() {
   else __syncthreads();
 }
 
+// Twiddle sources of a Stockham stage: the LDS table (w^(r k STEP)), or the
+// values this thread's butterflies need, held in registers (RegTw).
+template <typename T>
+struct TwLds {
+  const cx<T>* __restrict__ tw;
+  template <int S, int STEP>
+  __device__ __forceinline__ cx<T> get(int, int r, int k) const { return tw[r * k * STEP]; }
+};
+
+// The twiddles of every stage of NBT transforms of length N run by NT
+// threads, for this thread's butterflies: w[S][q][r] for butterfly q of
+// stage S.  A wave-owned row transform's butterflies are the same at every
+// level, so a row kernel loads them once instead of once per stage.
+template <typename T, int N, int NBT, int NT>
+struct RegTw {
+  static constexpr int NSTG = n_stages(N);
+  static constexpr int PERMAX = (NBT * (N / 2) + NT - 1) / NT;
+  cx<T> w[NSTG][PERMAX][8];
+  template <int S = 0>
+  __device__ __forceinline__ void init(const cx<T>* __restrict__ tw, int tid) {
+    if constexpr (S < NSTG) {
+      constexpr int R = radix_of(N, S), NS = span_before(N, S), NBF = N / R, TOT = NBT * NBF;
+      constexpr int PER = (TOT + NT - 1) / NT, STEP = N / (NS * R);
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int idx = tid + q * NT;
+          const int j = (idx < TOT ? idx : 0) % NBF;
+#pragma unroll
+          for (int r = 1; r < R; ++r) w[S][q][r] = tw[r * (j % NS) * STEP];
+        }
+      }
+      init<S + 1>(tw, tid);
+    }
+  }
+  template <int S, int STEP>
+  __device__ __forceinline__ cx<T> get(int q, int r, int) const { return w[S][q][r]; }
+};
+
+template <typename T>
+__device__ __forceinline__ TwLds<T> tw_src(const cx<T>* tw) { return TwLds<T>{tw}; }
+template <typename T, int N, int NBT, int NT>
+__device__ __forceinline__ const RegTw<T, N, NBT, NT>& tw_src(const RegTw<T, N, NBT, NT>& r) { return r; }
+
 // One Stockham stage over NBT transforms of length N held at a[t*N + n],
 // executed by NT threads (a whole workgroup, or one wavefront of 64 lanes
 // when WAVE).  Entry and exit are synchronised at that scope.
-template <typename T, int N, bool INV, int NBT, int NT, int S, bool WAVE, int PLD = padlen(N)>
-__device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
+template <typename T, int N, bool INV, int NBT, int NT, int S, bool WAVE, int PLD = padlen(N),
+          class TWS = TwLds<T>>
+__device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
   constexpr int R = radix_of(N, S);
   constexpr int NS = span_before(N, S);
   constexpr int NBF = N / R;
@@ -211,7 +257,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
         const int k = j % NS;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const cx<T> w = tw[r * k * STEP];
+          const cx<T> w = tws.template get<S, STEP>(q, r, k);
           v[q][r] = INV ? cmulc(v[q][r], w) : cmul(v[q][r], w);
         }
       }
@@ -235,12 +281,18 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const cx<T>* __restrict__ tw
   fft_sync<WAVE>();
 }
 
+template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),
+          class TWS>
+__device__ __forceinline__ void fft_run_src(cx<T>* a, const TWS& tws, int tid) {
+  if constexpr (S < n_stages(N)) {
+    fft_stage<T, N, INV, NBT, NT, S, WAVE, PLD, TWS>(a, tws, tid);
+    fft_run_src<T, N, INV, NBT, NT, S + 1, WAVE, PLD, TWS>(a, tws, tid);
+  }
+}
+
 template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N)>
 __device__ __forceinline__ void fft_run(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
-  if constexpr (S < n_stages(N)) {
-    fft_stage<T, N, INV, NBT, NT, S, WAVE, PLD>(a, tw, tid);
-    fft_run<T, N, INV, NBT, NT, S + 1, WAVE, PLD>(a, tw, tid);
-  }
+  fft_run_src<T, N, INV, NBT, NT, S, WAVE, PLD>(a, TwLds<T>{tw}, tid);
 }
 
 // NBT transforms (PLD elements apart, padded inside as pidx) spread over the

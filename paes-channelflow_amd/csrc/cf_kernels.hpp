@@ -572,8 +572,8 @@ __device__ __forceinline__ V pick(const V* arr, int q, V dflt) {
 // Forward-transform NP real row pairs (this thread holds point tid of each)
 // and hand coefficient k < MX of both members, scaled by 1/NXB, to
 // emit(q, k, A, B).
-template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE, typename Emit>
-__device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb, const cx<T>* tw,
+template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE, typename Emit, class TWA>
+__device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb, const TWA& tw,
                                            int tid, Emit emit) {
   using F = RowFFT<T, NX>;
   constexpr int MX = F::MX, NXB = F::NXB;
@@ -591,7 +591,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
   if constexpr (WV) {
     const int w = tid >> 6, lane = tid & 63;
     if (w * F::TPW < NP) {
-      fft_run<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw, lane);
+      fft_run_src<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw_src(tw), lane);
 #pragma unroll
       for (int qq = 0; qq < F::TPW; ++qq) {
         const int q = w * F::TPW + qq;
@@ -601,6 +601,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     }
     // the next synth_pairs only rewrites this wave's own transforms
   } else {
+    static_assert(!std::is_class<TWA>::value, "workgroup-shared transforms read the LDS table");
     fft_run<T, NXB, false, NP, F::NT>(buf, tw, tid);
     if (tid < MX) {
 #pragma unroll
@@ -639,10 +640,10 @@ __device__ __forceinline__ void row_prefetch(RowPre<T, MX, TPW>& P, const cx<T>*
 
 // synth_pairs (wave-owned transforms of length L) from prefetched half
 // spectra: the same spectrum, transform and result layout.
-template <typename T, int L, int MX, int TPW, int NP>
+template <typename T, int L, int MX, int TPW, int NP, class TWA>
 __device__ __forceinline__ void synth_pre(cx<T>* buf, const RowPre<T, MX, TPW>& P, const bool* da,
                                           const bool* db, T* ra, T* rb, const T* __restrict__ kxp,
-                                          const cx<T>* tw, int tid) {
+                                          const TWA& tw, int tid) {
   const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
   for (int qq = 0; qq < TPW; ++qq) {
@@ -670,7 +671,7 @@ __device__ __forceinline__ void synth_pre(cx<T>* buf, const RowPre<T, MX, TPW>& 
     }
   }
   wave_sync();
-  if (w * TPW < NP) fft_run<T, L, true, TPW, 64, 0, true>(buf + w * TPW * padlen(L), tw, lane);
+  if (w * TPW < NP) fft_run_src<T, L, true, TPW, 64, 0, true>(buf + w * TPW * padlen(L), tw_src(tw), lane);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
@@ -993,6 +994,18 @@ k_rows(RowArgs<T> a) {
   auto clampF = [&](int F) { return F <= 0 ? 1 : (F >= nz ? nz - 1 : F); };
   enum { FU = 0, FUY, FV, FVY, FW, FWY, FE, FEY };
   load_twiddles(tw, a.twNXB, NXB, tid, NT);
+  __syncthreads();                       // the table is read by every wave's transforms
+  // The lean FP32 sweep at nnx 256 holds each lane's twiddles of its
+  // wave-owned transform in registers, loaded once per kernel instead of once
+  // per transform stage (12 values; lean sweep 0.632 -> 0.612 ms).  The full
+  // sweep keeps the LDS table: the registers would spill there.
+  constexpr bool RT = LEAN && WV && rows_prefetch<T, NX>();
+  RegTw<T, NXB, F::TPW, 64> rtw;
+  if constexpr (RT) rtw.init(tw, tid & 63);
+  const auto& twx = [&]() -> const auto& {
+    if constexpr (RT) return rtw;
+    else return tw;
+  }();
 
   const T emin = (T)c.emin;
   const T alk = (T)c.alk, xkmax = (T)c.xkmax;
@@ -1081,7 +1094,7 @@ k_rows(RowArgs<T> a) {
       const bool da[3] = {false, false, true};
       const bool db[3] = {false, false, false};
       if constexpr (PRE) {
-        synth_pre<T, NXB, MX, F::TPW, 3>(buf, P, da, db, ra, rb, a.kxp, tw, tid);
+        synth_pre<T, NXB, MX, F::TPW, 3>(buf, P, da, db, ra, rb, a.kxp, twx, tid);
         if (iz + 1 < z1) prefetch(iz + 1);
       } else {
         const cx<T>* sa[6];
@@ -1102,7 +1115,7 @@ k_rows(RowArgs<T> a) {
       const bool da[6] = {false, false, true, true, true, true};
       const bool db[6] = {false, false, false, false, false, false};
       if constexpr (PRE) {
-        synth_pre<T, NXB, MX, F::TPW, 6>(buf, P, da, db, ra, rb, a.kxp, tw, tid);
+        synth_pre<T, NXB, MX, F::TPW, 6>(buf, P, da, db, ra, rb, a.kxp, twx, tid);
         if (iz + 1 < z1) prefetch(iz + 1);
       } else {
         const cx<T>* sa[6];
@@ -1224,7 +1237,7 @@ k_rows(RowArgs<T> a) {
           default: if (!top) A[7 * as + o + k] = fA; break;                    // Fye
         }
       };
-      anal_pairs<T, NX, 6, WV>(buf, fa, fb, tw, tid, emit);
+      anal_pairs<T, NX, 6, WV>(buf, fa, fb, twx, tid, emit);
     }
     if constexpr (WV) {
       // wave partials were published by the barriers inside anal_pairs
@@ -1294,6 +1307,7 @@ __device__ __forceinline__ void rows_mlp_body(const RowArgs<T>& a, const mlp::Ne
   enum { FU = 0, FUY, FV, FVY, FW, FWY, FE, FEY };
   load_twiddles(tw, a.twNXB, NXB, tid, NT);
   for (int q = tid; q < (int)(sizeof(mlp::Net) / 16); q += NT) ((uint4*)&net)[q] = ((const uint4*)gnet)[q];
+  __syncthreads();                       // the tables are read by every wave
 
   const T emin = (T)c.emin;
   const T alk = (T)c.alk, xkmax = (T)c.xkmax;
@@ -1771,6 +1785,7 @@ k_sgs_rows(SgsRowArgs<T> a, const mlp::Net* __restrict__ gnet) {
   };
   load_twiddles(tw, a.twNX, NX, tid, NT);
   for (int q = tid; q < (int)(sizeof(mlp::Net) / 16); q += NT) ((uint4*)&net)[q] = ((const uint4*)gnet)[q];
+  __syncthreads();                       // the tables are read by every wave
 
   const T alk = (T)c.alk, xkmax = (T)c.xkmax;
   const T dzm = (T)c.dzm, todzm = (T)c.todzm;
