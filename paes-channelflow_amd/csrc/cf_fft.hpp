@@ -234,8 +234,13 @@ __device__ __forceinline__ const RegTw<T, N, NBT, NT>& tw_src(const RegTw<T, N, 
 // One Stockham stage over NBT transforms of length N held at a[t*N + n],
 // executed by NT threads (a whole workgroup, or one wavefront of 64 lanes
 // when WAVE).  Entry and exit are synchronised at that scope.
+//
+// BAND > 0 marks a 3/2-grid transform whose elements BAND .. N-BAND are the
+// zero pad: an inverse transform's first stage takes them as zero without
+// reading them (the caller need not write them), and a forward transform's
+// last stage does not store them (the caller does not read them).
 template <typename T, int N, bool INV, int NBT, int NT, int S, bool WAVE, int PLD = padlen(N),
-          class TWS = TwLds<T>>
+          class TWS = TwLds<T>, int BAND = 0>
 __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
   constexpr int R = radix_of(N, S);
   constexpr int NS = span_before(N, S);
@@ -243,6 +248,8 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
   constexpr int TOT = NBT * NBF;
   constexpr int PER = (TOT + NT - 1) / NT;
   constexpr int STEP = N / (NS * R);
+  constexpr bool ZIN = INV && BAND > 0 && S == 0;
+  constexpr bool ZOUT = !INV && BAND > 0 && S == n_stages(N) - 1;   // there NS = NBF: output j + r NS
   cx<T> v[PER][R];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -252,7 +259,21 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
       const int j = idx - t * NBF;
       const cx<T>* src = a + t * PLD;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * NBF)];
+      for (int r = 0; r < R; ++r) {
+        const int lo = r * NBF, hi = r * NBF + NBF - 1;          // this r's input range
+        if (ZIN && lo >= BAND && hi <= N - BAND) {
+          v[q][r] = cx<T>{T(0), T(0)};                         // all of this r's inputs are pad
+        } else if (ZIN && hi >= BAND && lo <= N - BAND) {
+          // partly pad: every lane loads (an in-bounds, possibly unwritten
+          // element) and the pad lanes select zero -- no exec-masked load
+          const int p = j + lo;
+          const cx<T> x = src[pidx(p)];
+          const bool pad = p >= BAND && p <= N - BAND;
+          v[q][r] = cx<T>{pad ? T(0) : x.x, pad ? T(0) : x.y};
+        } else {
+          v[q][r] = src[pidx(j + r * NBF)];
+        }
+      }
       if constexpr (NS > 1) {
         const int k = j % NS;
 #pragma unroll
@@ -275,38 +296,42 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
       cx<T>* dst = a + t * PLD;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[pidx(base + r * NS)] = v[q][r];
+      for (int r = 0; r < R; ++r) {
+        if (ZOUT && r * NS >= BAND && r * NS + NS - 1 <= N - BAND) continue;
+        dst[pidx(base + r * NS)] = v[q][r];
+      }
     }
   }
   fft_sync<WAVE>();
 }
 
 template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),
-          class TWS>
+          int BAND = 0, class TWS>
 __device__ __forceinline__ void fft_run_src(cx<T>* a, const TWS& tws, int tid) {
   if constexpr (S < n_stages(N)) {
-    fft_stage<T, N, INV, NBT, NT, S, WAVE, PLD, TWS>(a, tws, tid);
-    fft_run_src<T, N, INV, NBT, NT, S + 1, WAVE, PLD, TWS>(a, tws, tid);
+    fft_stage<T, N, INV, NBT, NT, S, WAVE, PLD, TWS, BAND>(a, tws, tid);
+    fft_run_src<T, N, INV, NBT, NT, S + 1, WAVE, PLD, BAND, TWS>(a, tws, tid);
   }
 }
 
-template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N)>
+template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),
+          int BAND = 0>
 __device__ __forceinline__ void fft_run(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
-  fft_run_src<T, N, INV, NBT, NT, S, WAVE, PLD>(a, TwLds<T>{tw}, tid);
+  fft_run_src<T, N, INV, NBT, NT, S, WAVE, PLD, BAND>(a, TwLds<T>{tw}, tid);
 }
 
 // NBT transforms (PLD elements apart, padded inside as pidx) spread over the
 // NW wavefronts of a workgroup, each wave running its share with wave-level
 // ordering only.  The caller synchronises the workgroup before (inputs
 // written) and after (outputs read by others).
-template <typename T, int N, bool INV, int NBT, int NW, int PLD = padlen(N)>
+template <typename T, int N, bool INV, int NBT, int NW, int PLD = padlen(N), int BAND = 0>
 __device__ __forceinline__ void fft_waves(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {
   const int w = tid >> 6, lane = tid & 63;
   if constexpr (NBT % NW == 0) {
     constexpr int PW = NBT / NW;
-    fft_run<T, N, INV, PW, 64, 0, true, PLD>(a + w * PW * PLD, tw, lane);
+    fft_run<T, N, INV, PW, 64, 0, true, PLD, BAND>(a + w * PW * PLD, tw, lane);
   } else {
-    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true, PLD>(a + t * PLD, tw, lane);
+    for (int t = w; t < NBT; t += NW) fft_run<T, N, INV, 1, 64, 0, true, PLD, BAND>(a + t * PLD, tw, lane);
   }
 }
 

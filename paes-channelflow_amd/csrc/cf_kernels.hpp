@@ -380,7 +380,6 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TKC, NT = Sh::NT;
   constexpr int PLB = colpad(NYB), PLS = colpad(NY);
   constexpr int LPS = (TK * NY + NT - 1) / NT;        // band values per thread
-  constexpr int NZP = NYB - NY + 1;                    // zero modes NY/2 .. NYB - NY/2
   static_assert(PLS <= PLB, "the length-NY batch fits the length-NYB batch");
   const int f = 3 - (int)blockIdx.z;     // e first: its tiles carry three transforms, u, v, w one or two
   const int z = blockIdx.y + 1;
@@ -441,12 +440,8 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
         }
       }
     }
-    for (int i = tid; i < TK * NZP; i += NT) {
-      const int c = i / NZP;
-      buf[c * PLB + pidx(NY / 2 + (i - c * NZP))] = cx<T>{0, 0};
-    }
-    __syncthreads();
-    fft_waves<T, NYB, true, TK, NT / 64, PLB>(buf, twb, tid);
+    __syncthreads();                     // the zero pad NY/2 .. NYB-NY/2 is not read (BAND)
+    fft_waves<T, NYB, true, TK, NT / 64, PLB, NY / 2>(buf, twb, tid);
     __syncthreads();
     cx<T>* od = o + (size_t)d * a.bstride;
 #pragma unroll
@@ -502,7 +497,9 @@ struct RowFFT {
 
 // Build the full length-NXB spectrum whose inverse DFT is a + i b for two real
 // rows with half spectra sa (a) and sb (b), k < MX; optional i kx factors.
-template <typename T, int NX>
+// ZFILL = false leaves the zero pad MX .. NXB-MX unwritten, for a transform
+// that takes it as zero (fft_run's BAND).
+template <typename T, int NX, bool ZFILL = true>
 __device__ __forceinline__ void fill_pair(cx<T>* z, const cx<T>* sa, bool da, const cx<T>* sb,
                                           bool db, const T* __restrict__ kxp, int t, int nt) {
   constexpr int MX = NX / 2, NXB = 3 * NX / 2;
@@ -520,7 +517,8 @@ __device__ __forceinline__ void fill_pair(cx<T>* z, const cx<T>* sa, bool da, co
       z[pidx(NXB - k)] = cx<T>{av.x + bv.y, bv.x - av.y};
     }
   }
-  for (int i = t; i <= MX; i += nt) z[pidx(MX + i)] = cx<T>{0, 0};
+  if constexpr (ZFILL)
+    for (int i = t; i <= MX; i += nt) z[pidx(MX + i)] = cx<T>{0, 0};
 }
 
 // Synthesise NP (<= 6) real row pairs; this thread's point of the two real
@@ -540,15 +538,17 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
 #pragma unroll
     for (int qq = 0; qq < F::TPW; ++qq) {
       const int q = w * F::TPW + qq;
-      if (q < NP) fill_pair<T, NX>(buf + q * padlen(NXB), sa[q], da[q], sb[q], db[q], kxp, lane, 64);
+      if (q < NP) fill_pair<T, NX, false>(buf + q * padlen(NXB), sa[q], da[q], sb[q], db[q], kxp, lane, 64);
     }
     wave_sync();
-    if (w * F::TPW < NP) fft_run<T, NXB, true, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw, lane);
+    if (w * F::TPW < NP)
+      fft_run<T, NXB, true, F::TPW, 64, 0, true, padlen(NXB), F::MX>(buf + w * F::TPW * padlen(NXB), tw, lane);
   } else {
 #pragma unroll
-    for (int q = 0; q < NP; ++q) fill_pair<T, NX>(buf + q * padlen(NXB), sa[q], da[q], sb[q], db[q], kxp, tid, F::NT);
+    for (int q = 0; q < NP; ++q)
+      fill_pair<T, NX, false>(buf + q * padlen(NXB), sa[q], da[q], sb[q], db[q], kxp, tid, F::NT);
     __syncthreads();
-    fft_run<T, NXB, true, NP, F::NT>(buf, tw, tid);
+    fft_run<T, NXB, true, NP, F::NT, 0, false, padlen(NXB), F::MX>(buf, tw, tid);
   }
   __syncthreads();
 #pragma unroll
@@ -591,7 +591,8 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
   if constexpr (WV) {
     const int w = tid >> 6, lane = tid & 63;
     if (w * F::TPW < NP) {
-      fft_run_src<T, NXB, false, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NXB), tw_src(tw), lane);
+      fft_run_src<T, NXB, false, F::TPW, 64, 0, true, padlen(NXB), MX>(buf + w * F::TPW * padlen(NXB), tw_src(tw),
+                                                                     lane);
 #pragma unroll
       for (int qq = 0; qq < F::TPW; ++qq) {
         const int q = w * F::TPW + qq;
@@ -602,7 +603,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     // the next synth_pairs only rewrites this wave's own transforms
   } else {
     static_assert(!std::is_class<TWA>::value, "workgroup-shared transforms read the LDS table");
-    fft_run<T, NXB, false, NP, F::NT>(buf, tw, tid);
+    fft_run<T, NXB, false, NP, F::NT, 0, false, padlen(NXB), MX>(buf, tw, tid);
     if (tid < MX) {
 #pragma unroll
       for (int q = 0; q < NP; ++q) extract(q, tid);
@@ -667,11 +668,11 @@ __device__ __forceinline__ void synth_pre(cx<T>* buf, const RowPre<T, MX, TPW>& 
           }
         }
       }
-      for (int i = MX + lane; i <= L - MX; i += 64) z[pidx(i)] = cx<T>{0, 0};
     }
   }
   wave_sync();
-  if (w * TPW < NP) fft_run_src<T, L, true, TPW, 64, 0, true>(buf + w * TPW * padlen(L), tw_src(tw), lane);
+  if (w * TPW < NP)
+    fft_run_src<T, L, true, TPW, 64, 0, true, padlen(L), MX>(buf + w * TPW * padlen(L), tw_src(tw), lane);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
