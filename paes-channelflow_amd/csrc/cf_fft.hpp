@@ -305,13 +305,37 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
   fft_sync<WAVE>();
 }
 
+// Stages S .. SE-1 (SE defaults to all of them).
 template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),
-          int BAND = 0, class TWS>
+          int BAND = 0, int SE = n_stages(N), class TWS>
 __device__ __forceinline__ void fft_run_src(cx<T>* a, const TWS& tws, int tid) {
-  if constexpr (S < n_stages(N)) {
+  if constexpr (S < SE) {
     fft_stage<T, N, INV, NBT, NT, S, WAVE, PLD, TWS, BAND>(a, tws, tid);
-    fft_run_src<T, N, INV, NBT, NT, S + 1, WAVE, PLD, BAND, TWS>(a, tws, tid);
+    fft_run_src<T, N, INV, NBT, NT, S + 1, WAVE, PLD, BAND, SE, TWS>(a, tws, tid);
   }
+}
+
+// The last stage of one wave-owned transform of length N whose last stage has
+// one butterfly per lane (N = 64 R): lane j reads x[j + 64 r], r < R, applies
+// its twiddles and the radix-R butterfly, and keeps the outputs X[j + 64 r] in
+// v[r] instead of storing them (the stages before it ran with SE =
+// n_stages(N) - 1).
+template <typename T, int N>
+constexpr int last_radix() { return radix_of(N, n_stages(N) - 1); }
+template <typename T, int N, bool INV, class TWS>
+__device__ __forceinline__ void fft_last_regs(const cx<T>* a, const TWS& tws, int lane,
+                                              cx<T> (&v)[last_radix<T, N>()]) {
+  constexpr int S = n_stages(N) - 1;
+  constexpr int R = radix_of(N, S), NS = span_before(N, S), NBF = N / R, STEP = N / (NS * R);
+  static_assert(NBF == 64 && NS == 64, "one butterfly per lane");
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = a[pidx(lane + r * NBF)];
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    const cx<T> w = tws.template get<S, STEP>(0, r, lane);
+    v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+  }
+  butterfly<T, R, INV>(v);
 }
 
 template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),

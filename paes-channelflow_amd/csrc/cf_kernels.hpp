@@ -588,7 +588,37 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     const cx<T> zn = buf[q * padlen(NXB) + pidx(k == 0 ? 0 : NXB - k)];
     emit(q, k, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s}, cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
   };
-  if constexpr (WV) {
+  constexpr int RL = last_radix<T, NXB>();
+  if constexpr (WV && F::TPW == 1 && NXB == 64 * RL && MX % 64 == 0) {
+    // one transform per wave whose last stage has one butterfly per lane:
+    // that stage stays in registers (lane j holds X[j + 64 r]); the mirror
+    // coefficient X[NXB - k] of k = j + 64 kk is X[64 (RL-1-kk) + 64 - j], held
+    // by lane 64 - j (k = 64 kk of lane 0: lane 0's own X[64 (RL-kk)], and
+    // X[0] for k = 0)
+    constexpr int KK = MX / 64;
+    const int w = tid >> 6, lane = tid & 63;
+    if (w < NP) {
+      cx<T>* z = buf + w * padlen(NXB);
+      fft_run_src<T, NXB, false, 1, 64, 0, true, padlen(NXB), MX, n_stages(NXB) - 1>(z, tw_src(tw), lane);
+      cx<T> v[RL];
+      fft_last_regs<T, NXB, false>(z, tw_src(tw), lane, v);
+      const int src = (64 - lane) & 63;
+      cx<T> m[KK];
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk) {
+        const cx<T> x = v[RL - 1 - kk];
+        m[kk] = cx<T>{__shfl(x.x, src, 64), __shfl(x.y, src, 64)};
+      }
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk) {
+        const cx<T> zk = v[kk];
+        const cx<T> zn = lane != 0 ? m[kk] : (kk == 0 ? v[0] : m[kk - 1]);
+        emit(w, lane + 64 * kk, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s},
+             cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
+      }
+    }
+    // the next synth_pairs only rewrites this wave's own transforms
+  } else if constexpr (WV) {
     const int w = tid >> 6, lane = tid & 63;
     if (w * F::TPW < NP) {
       fft_run_src<T, NXB, false, F::TPW, 64, 0, true, padlen(NXB), MX>(buf + w * F::TPW * padlen(NXB), tw_src(tw),
