@@ -338,6 +338,81 @@ __device__ __forceinline__ void fft_last_regs(const cx<T>* a, const TWS& tws, in
   butterfly<T, R, INV>(v);
 }
 
+// The inverse transform in transposed form.  The forward DFT matrix is
+// symmetric, so for the forward stages F = S_{n-1} ... S_0 the unnormalised
+// inverse is conj(F) = conj(S_0^T) ... conj(S_{n-1}^T): the stages run last
+// to first, and stage S transposed gathers from stage S's output positions,
+// runs the inverse butterfly, multiplies by the conjugate twiddles and
+// scatters to stage S's input positions.  It needs the forward transform's
+// twiddle set (the same RegTw registers), and its first stage is the forward
+// transform's last one: for N = 64 R one butterfly per lane on x[j + 64 r],
+// which a caller can feed from registers (fft_first_regs_t).
+template <typename T, int N, int NBT, int NT, int S, bool WAVE, int PLD, class TWS>
+__device__ __forceinline__ void fft_stage_t(cx<T>* a, const TWS& tws, int tid) {
+  constexpr int R = radix_of(N, S);
+  constexpr int NS = span_before(N, S);
+  constexpr int NBF = N / R;
+  constexpr int TOT = NBT * NBF;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  constexpr int STEP = N / (NS * R);
+  cx<T> v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = tid + q * NT;
+    if ((TOT % NT) == 0 || idx < TOT) {
+      const int t = idx / NBF;
+      const int j = idx - t * NBF;
+      const int k = j % NS;
+      const cx<T>* src = a + t * PLD;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = src[pidx(base + r * NS)];
+      butterfly<T, R, true>(v[q]);
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = cmulc(v[q][r], tws.template get<S, STEP>(q, r, k));
+      }
+    }
+  }
+  fft_sync<WAVE>();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = tid + q * NT;
+    if ((TOT % NT) == 0 || idx < TOT) {
+      const int t = idx / NBF;
+      const int j = idx - t * NBF;
+      cx<T>* dst = a + t * PLD;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[pidx(j + r * NBF)] = v[q][r];
+    }
+  }
+  fft_sync<WAVE>();
+}
+
+// Transposed stages S, S-1, .., 0.
+template <typename T, int N, int NBT, int NT, int S, bool WAVE, int PLD, class TWS>
+__device__ __forceinline__ void fft_run_t(cx<T>* a, const TWS& tws, int tid) {
+  fft_stage_t<T, N, NBT, NT, S, WAVE, PLD, TWS>(a, tws, tid);
+  if constexpr (S > 0) fft_run_t<T, N, NBT, NT, S - 1, WAVE, PLD, TWS>(a, tws, tid);
+}
+
+// The first transposed stage of one wave-owned inverse transform of length
+// N = 64 R from registers: lane j holds x[j + 64 r] in v[r]; inverse
+// butterfly, conjugate twiddles, outputs stored at a[j + 64 r].  The caller
+// orders the stores (wave_sync) before fft_run_t of stages n_stages(N)-2 .. 0.
+template <typename T, int N, class TWS>
+__device__ __forceinline__ void fft_first_regs_t(cx<T>* a, const TWS& tws, int lane,
+                                                 cx<T> (&v)[last_radix<T, N>()]) {
+  constexpr int S = n_stages(N) - 1;
+  constexpr int R = radix_of(N, S), NS = span_before(N, S), NBF = N / R, STEP = N / (NS * R);
+  static_assert(NBF == 64 && NS == 64, "one butterfly per lane");
+  butterfly<T, R, true>(v);
+#pragma unroll
+  for (int r = 1; r < R; ++r) v[r] = cmulc(v[r], tws.template get<S, STEP>(0, r, lane));
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[pidx(lane + r * NBF)] = v[r];
+}
+
 template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),
           int BAND = 0>
 __device__ __forceinline__ void fft_run(cx<T>* a, const cx<T>* __restrict__ tw, int tid) {

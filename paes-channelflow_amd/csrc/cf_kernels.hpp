@@ -521,6 +521,68 @@ __device__ __forceinline__ void fill_pair(cx<T>* z, const cx<T>* sa, bool da, co
     for (int i = t; i <= MX; i += nt) z[pidx(MX + i)] = cx<T>{0, 0};
 }
 
+// Element q (wave-uniform, not a constant) of a small local array, by
+// compare-and-select: indexing it directly would place the array in scratch.
+template <int NP, typename V>
+__device__ __forceinline__ V pick(const V* arr, int q, V dflt) {
+  V r = dflt;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) r = (j == q) ? arr[j] : r;
+  return r;
+}
+
+// Whether the synthesis of a wave-owned length-L transform of half spectra
+// k < MX can start from registers: L = 64 R for its first transposed stage
+// (one butterfly per lane), and MX a multiple of 64 so that every stride
+// class j + 64 r is all data, all pad or all mirrored (lane 0 aside).
+template <typename T, int L, int MX>
+constexpr bool synth_in_regs() {
+  return L == 64 * last_radix<T, L>() && MX % 64 == 0 && 2 * MX <= L;
+}
+
+// NB wave-owned inverse transforms of length L (PLD elements apart) whose
+// spectra Z are built from the half spectra of two real rows a, b each: lane j
+// holds modes k = j + 64 kk < MX of transform t in av[t][kk], bv[t][kk] (i kx
+// factors applied).  Z[k] = A[k] + i B[k] for k < MX, Z[L - k] = conj(A[k]) +
+// i conj(B[k]), zero in between (the 3/2-grid pad, or the Nyquist mode of the
+// small grid).  The first transposed stage reads Z[j + 64 r] from registers:
+// classes r below MX are the lane's own modes, classes above L - MX the
+// mirrors of lane 64 - j's (lane 0: its own, or zero at k = L - MX), the rest
+// zero.  The inverse DFTs of the Z are left in z.
+template <typename T, int L, int MX, int NB, int PLD, class TWS>
+__device__ __forceinline__ void synth_regs(cx<T>* z, const cx<T> (&av)[NB][MX / 64],
+                                           const cx<T> (&bv)[NB][MX / 64], const TWS& tws, int lane) {
+  constexpr int KPL = MX / 64, R = last_radix<T, L>();
+  static_assert(synth_in_regs<T, L, MX>(), "one butterfly per lane over whole stride classes");
+  const int src = (64 - lane) & 63;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    cx<T> v[R], mir[KPL];
+#pragma unroll
+    for (int kk = 0; kk < KPL; ++kk) {
+      const cx<T> a = av[t][kk], b = bv[t][kk];
+      // c2r ignores the imaginary part of the DC term
+      v[kk] = (kk == 0 && lane == 0) ? cx<T>{a.x, b.x} : cx<T>{a.x - b.y, a.y + b.x};
+      mir[kk] = cx<T>{a.x + b.y, b.x - a.y};
+    }
+#pragma unroll
+    for (int r = KPL; r < R; ++r) {
+      if (64 * r + 63 <= L - MX) {
+        v[r] = cx<T>{T(0), T(0)};
+      } else {
+        const int m0 = (L - 64 * r) / 64;           // lane j >= 1: mode 64 (m0 - 1) + 64 - j
+        const cx<T> x = mir[m0 - 1];
+        const cx<T> m = {__shfl(x.x, src, 64), __shfl(x.y, src, 64)};
+        const cx<T> own = (64 * m0 < MX) ? mir[m0 < KPL ? m0 : 0] : cx<T>{T(0), T(0)};
+        v[r] = lane != 0 ? m : own;
+      }
+    }
+    fft_first_regs_t<T, L>(z + t * PLD, tws, lane, v);
+  }
+  wave_sync();
+  fft_run_t<T, L, NB, 64, n_stages(L) - 2, true, PLD>(z, tws, lane);
+}
+
 // Synthesise NP (<= 6) real row pairs; this thread's point of the two real
 // rows of pair q comes back in ra[q], rb[q].  Ends with a workgroup barrier
 // after which the caller must not write buf until the next anal_pairs.
@@ -533,7 +595,27 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   using F = RowFFT<T, NX>;
   constexpr int NXB = F::NXB;
   static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
-  if constexpr (WV) {
+  if constexpr (WV && F::TPW == 1 && synth_in_regs<T, NXB, F::MX>()) {
+    const int w = tid >> 6, lane = tid & 63;
+    if (w < NP) {
+      const cx<T>* pa = pick<NP>(sa, w, (const cx<T>*)nullptr);
+      const cx<T>* pb = pick<NP>(sb, w, (const cx<T>*)nullptr);
+      const bool dqa = pick<NP>(da, w, false), dqb = pick<NP>(db, w, false);
+      cx<T> av[1][F::MX / 64], bv[1][F::MX / 64];
+#pragma unroll
+      for (int kk = 0; kk < F::MX / 64; ++kk) {
+        const int k = lane + 64 * kk;
+        cx<T> x = pa ? pa[k] : cx<T>{T(0), T(0)};
+        cx<T> y = pb ? pb[k] : cx<T>{T(0), T(0)};
+        const T kx = kxp[k];
+        if (dqa) x = cx<T>{-kx * x.y, kx * x.x};
+        if (dqb) y = cx<T>{-kx * y.y, kx * y.x};
+        av[0][kk] = x;
+        bv[0][kk] = y;
+      }
+      synth_regs<T, NXB, F::MX, 1, padlen(NXB)>(buf + w * padlen(NXB), av, bv, tw_src(tw), lane);
+    }
+  } else if constexpr (WV) {
     const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
     for (int qq = 0; qq < F::TPW; ++qq) {
@@ -559,14 +641,30 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   }
 }
 
-// Element q (wave-uniform, not a constant) of a small local array, by
-// compare-and-select: indexing it directly would place the array in scratch.
-template <int NP, typename V>
-__device__ __forceinline__ V pick(const V* arr, int q, V dflt) {
-  V r = dflt;
+// Coefficients k < MX of the two real rows of transform q from the outputs of
+// a wave-owned forward transform of length L = 64 R kept in registers (lane j
+// holds X[j + 64 r] in v[r]), scaled by s, handed to emit(q, k, A, B).  The
+// mirror X[L - k] of k = j + 64 kk is X[64 (R-1-kk) + 64 - j], held by lane
+// 64 - j (k = 64 kk of lane 0: lane 0's own X[64 (R-kk)], and X[0] for k = 0).
+template <typename T, int L, int MX, typename Emit>
+__device__ __forceinline__ void emit_from_regs(const cx<T> (&v)[last_radix<T, L>()], int lane, T s, int q,
+                                               Emit& emit) {
+  constexpr int R = last_radix<T, L>(), KK = MX / 64;
+  static_assert(L == 64 * R && MX % 64 == 0 && 2 * MX <= L, "one butterfly per lane, whole classes");
+  const int src = (64 - lane) & 63;
+  cx<T> m[KK];
 #pragma unroll
-  for (int j = 0; j < NP; ++j) r = (j == q) ? arr[j] : r;
-  return r;
+  for (int kk = 0; kk < KK; ++kk) {
+    const cx<T> x = v[R - 1 - kk];
+    m[kk] = cx<T>{__shfl(x.x, src, 64), __shfl(x.y, src, 64)};
+  }
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk) {
+    const cx<T> zk = v[kk];
+    const cx<T> zn = lane != 0 ? m[kk] : (kk == 0 ? v[0] : m[kk - 1]);
+    emit(q, lane + 64 * kk, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s},
+         cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
+  }
 }
 
 // Forward-transform NP real row pairs (this thread holds point tid of each)
@@ -595,27 +693,13 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     // coefficient X[NXB - k] of k = j + 64 kk is X[64 (RL-1-kk) + 64 - j], held
     // by lane 64 - j (k = 64 kk of lane 0: lane 0's own X[64 (RL-kk)], and
     // X[0] for k = 0)
-    constexpr int KK = MX / 64;
     const int w = tid >> 6, lane = tid & 63;
     if (w < NP) {
       cx<T>* z = buf + w * padlen(NXB);
       fft_run_src<T, NXB, false, 1, 64, 0, true, padlen(NXB), MX, n_stages(NXB) - 1>(z, tw_src(tw), lane);
       cx<T> v[RL];
       fft_last_regs<T, NXB, false>(z, tw_src(tw), lane, v);
-      const int src = (64 - lane) & 63;
-      cx<T> m[KK];
-#pragma unroll
-      for (int kk = 0; kk < KK; ++kk) {
-        const cx<T> x = v[RL - 1 - kk];
-        m[kk] = cx<T>{__shfl(x.x, src, 64), __shfl(x.y, src, 64)};
-      }
-#pragma unroll
-      for (int kk = 0; kk < KK; ++kk) {
-        const cx<T> zk = v[kk];
-        const cx<T> zn = lane != 0 ? m[kk] : (kk == 0 ? v[0] : m[kk - 1]);
-        emit(w, lane + 64 * kk, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s},
-             cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
-      }
+      emit_from_regs<T, NXB, MX>(v, lane, s, w, emit);
     }
     // the next synth_pairs only rewrites this wave's own transforms
   } else if constexpr (WV) {
@@ -676,6 +760,31 @@ __device__ __forceinline__ void synth_pre(cx<T>* buf, const RowPre<T, MX, TPW>& 
                                           const bool* db, T* ra, T* rb, const T* __restrict__ kxp,
                                           const TWA& tw, int tid) {
   const int w = tid >> 6, lane = tid & 63;
+  if constexpr (TPW == 1 && synth_in_regs<T, L, MX>()) {
+    if (w < NP) {
+      const bool dqa = pick<NP>(da, w, false), dqb = pick<NP>(db, w, false);
+      cx<T> av[1][MX / 64], bv[1][MX / 64];
+#pragma unroll
+      for (int kk = 0; kk < MX / 64; ++kk) {
+        const int k = lane + 64 * kk;
+        cx<T> x = P.a[0][kk], y = P.b[0][kk];
+        const T kx = kxp[k];
+        if (dqa) x = cx<T>{-kx * x.y, kx * x.x};
+        if (dqb) y = cx<T>{-kx * y.y, kx * y.x};
+        av[0][kk] = x;
+        bv[0][kk] = y;
+      }
+      synth_regs<T, L, MX, 1, padlen(L)>(buf + w * padlen(L), av, bv, tw_src(tw), lane);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const cx<T> v = buf[q * padlen(L) + pidx(tid)];
+      ra[q] = v.x;
+      rb[q] = v.y;
+    }
+    return;
+  }
 #pragma unroll
   for (int qq = 0; qq < TPW; ++qq) {
     const int q = w * TPW + qq;
@@ -1662,13 +1771,38 @@ __device__ __forceinline__ void synth_small(cx<T>* buf, const cx<T>* const* sa, 
                                             const T* __restrict__ kxp, const cx<T>* tw, int tid) {
   using F = SmallRowFFT<T, NX>;
   const int w = tid >> 6, lane = tid & 63;
+  if constexpr (synth_in_regs<T, NX, F::MX>()) {
+    if (w * F::TPW < NP) {
+      cx<T> av[F::TPW][F::MX / 64], bv[F::TPW][F::MX / 64];
 #pragma unroll
-  for (int qq = 0; qq < F::TPW; ++qq) {
-    const int q = w * F::TPW + qq;
-    if (q < NP) fill_pair_len<T, NX, F::MX>(buf + q * padlen(NX), sa[q], da[q], sb[q], db[q], kxp, lane, 64);
+      for (int qq = 0; qq < F::TPW; ++qq) {
+        const int q = w * F::TPW + qq;
+        const cx<T>* pa = pick<NP>(sa, q, (const cx<T>*)nullptr);
+        const cx<T>* pb = pick<NP>(sb, q, (const cx<T>*)nullptr);
+        const bool dqa = pick<NP>(da, q, false), dqb = pick<NP>(db, q, false);
+#pragma unroll
+        for (int kk = 0; kk < F::MX / 64; ++kk) {
+          const int k = lane + 64 * kk;
+          cx<T> x = pa ? pa[k] : cx<T>{T(0), T(0)};
+          cx<T> y = pb ? pb[k] : cx<T>{T(0), T(0)};
+          const T kx = kxp[k];
+          if (dqa) x = cx<T>{-kx * x.y, kx * x.x};
+          if (dqb) y = cx<T>{-kx * y.y, kx * y.x};
+          av[qq][kk] = x;
+          bv[qq][kk] = y;
+        }
+      }
+      synth_regs<T, NX, F::MX, F::TPW, padlen(NX)>(buf + w * F::TPW * padlen(NX), av, bv, tw_src(tw), lane);
+    }
+  } else {
+#pragma unroll
+    for (int qq = 0; qq < F::TPW; ++qq) {
+      const int q = w * F::TPW + qq;
+      if (q < NP) fill_pair_len<T, NX, F::MX>(buf + q * padlen(NX), sa[q], da[q], sb[q], db[q], kxp, lane, 64);
+    }
+    wave_sync();
+    if (w * F::TPW < NP) fft_run<T, NX, true, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NX), tw, lane);
   }
-  wave_sync();
-  if (w * F::TPW < NP) fft_run<T, NX, true, F::TPW, 64, 0, true>(buf + w * F::TPW * padlen(NX), tw, lane);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
@@ -1716,6 +1850,35 @@ __device__ __forceinline__ void synth_small_pre(cx<T>* buf, const SmallPre<T, NX
   using F = SmallRowFFT<T, NX>;
   constexpr int L = NX, MX = F::MX;
   const int w = tid >> 6, lane = tid & 63;
+  if constexpr (synth_in_regs<T, L, MX>()) {
+    if (w * F::TPW < NP) {
+      cx<T> av[F::TPW][MX / 64], bv[F::TPW][MX / 64];
+#pragma unroll
+      for (int qq = 0; qq < F::TPW; ++qq) {
+        const int q = w * F::TPW + qq;
+        const bool dqa = pick<NP>(da, q, false), dqb = pick<NP>(db, q, false);
+#pragma unroll
+        for (int kk = 0; kk < MX / 64; ++kk) {
+          const int k = lane + 64 * kk;
+          cx<T> x = P.a[qq][kk], y = P.b[qq][kk];
+          const T kx = kxp[k];
+          if (dqa) x = cx<T>{-kx * x.y, kx * x.x};
+          if (dqb) y = cx<T>{-kx * y.y, kx * y.x};
+          av[qq][kk] = x;
+          bv[qq][kk] = y;
+        }
+      }
+      synth_regs<T, L, MX, F::TPW, padlen(NX)>(buf + w * F::TPW * padlen(NX), av, bv, tw_src(tw), lane);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const cx<T> v = buf[q * padlen(NX) + pidx(tid)];
+      ra[q] = v.x;
+      rb[q] = v.y;
+    }
+    return;
+  }
 #pragma unroll
   for (int qq = 0; qq < F::TPW; ++qq) {
     const int q = w * F::TPW + qq;
@@ -1765,7 +1928,17 @@ __device__ __forceinline__ void anal_small(cx<T>* buf, const T* xa, const T* xb,
   for (int q = 0; q < NP; ++q) buf[q * padlen(NX) + pidx(tid)] = cx<T>{xa[q], xb[q]};
   __syncthreads();
   const int w = tid >> 6, lane = tid & 63;
-  if (w * TPWA < NP) {
+  constexpr int RL = last_radix<T, NX>();
+  if constexpr (TPWA == 1 && NX == 64 * RL && MX % 64 == 0) {
+    // the last stage stays in registers, as in anal_pairs
+    if (w < NP) {
+      cx<T>* z = buf + w * padlen(NX);
+      fft_run_src<T, NX, false, 1, 64, 0, true, padlen(NX), 0, n_stages(NX) - 1>(z, tw_src(tw), lane);
+      cx<T> v[RL];
+      fft_last_regs<T, NX, false>(z, tw_src(tw), lane, v);
+      emit_from_regs<T, NX, MX>(v, lane, s, w, emit);
+    }
+  } else if (w * TPWA < NP) {
     fft_run<T, NX, false, TPWA, 64, 0, true>(buf + w * TPWA * padlen(NX), tw, lane);
 #pragma unroll
     for (int qq = 0; qq < TPWA; ++qq) {
