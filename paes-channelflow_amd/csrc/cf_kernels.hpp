@@ -46,6 +46,14 @@ struct ColShape {
   // k_col_synth / k_col_anal: at least one transform per wavefront of the
   // 512-thread workgroup (they pass one batch of TKC transforms through LDS)
   static constexpr int TKC = TK >= 8 ? TK : (MX < 8 ? MX : 8);
+  // Column stages from registers: with 64 threads per column of a TKC tile
+  // and NY = 256, thread (c, j0) = (tid % TKC, tid / TKC) holds band rows
+  // j0 + 64 q, q < 4.  On the 3/2 grid (384 = 64 x 6) those are stride-64
+  // classes 0, 1, 4, 5 of index j0 (2 and 3 are pad, and so is class 4 at
+  // j0 = 0, the Nyquist row); on the small grid (256 = 64 x 4) classes 0..3.
+  // A first transposed inverse stage or a last forward stage (one butterfly
+  // per index j0 of a column) then runs on exactly the thread's own values.
+  static constexpr bool CREG = NY == 256 && 512 / TKC == 64;
   // Eight waves per tile.  The column kernels pass one batch of TK padded
   // transforms through LDS at a time (about 32 KB at 256^2 FP32), so four
   // workgroups share a CU.
@@ -407,15 +415,28 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
       buf[c * PLS + pidx(y)] = src[(size_t)y * MX + c];
     }
     __syncthreads();
-    fft_waves<T, NY, false, TK, NT / 64, PLS>(buf, tws, tid);
-    __syncthreads();
     const T s = T(1) / T(NY);
+    if constexpr (Sh::CREG) {
+      // stages before the last per owning wave, the last one on the thread
+      // that keeps the band rows j0 + 64 q (classes q of the length-256 grid)
+      const int w = tid >> 6, lane = tid & 63, c = tid % TK, j0 = tid / TK;
+      fft_run_src<T, NY, false, TK / (NT / 64), 64, 0, true, PLS, 0, n_stages(NY) - 1>(
+          buf + w * (TK / (NT / 64)) * PLS, tw_src(tws), lane);
+      __syncthreads();
+      cx<T> x[4];
+      fft_last_regs<T, NY, false>(buf + c * PLS, tw_src(tws), j0, x);
 #pragma unroll
-    for (int q = 0; q < LPS; ++q) {
-      const int idx = tid + q * NT;
-      if (idx < TK * NY) {
-        const int c = idx % TK, j = idx / TK;
-        sv[q] = (j == NY / 2) ? cx<T>{0, 0} : cscale(buf[c * PLS + pidx(j)], s);
+      for (int q = 0; q < 4; ++q) sv[q] = (q == 2 && j0 == 0) ? cx<T>{0, 0} : cscale(x[q], s);
+    } else {
+      fft_waves<T, NY, false, TK, NT / 64, PLS>(buf, tws, tid);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < LPS; ++q) {
+        const int idx = tid + q * NT;
+        if (idx < TK * NY) {
+          const int c = idx % TK, j = idx / TK;
+          sv[q] = (j == NY / 2) ? cx<T>{0, 0} : cscale(buf[c * PLS + pidx(j)], s);
+        }
       }
     }
     __syncthreads();                     // buf is refilled below
@@ -425,23 +446,46 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   // loop: a loop lets the compiler hoist the transform's twiddle reads out of
   // it and hold them in registers across both passes.
   auto pass = [&](int d) {
+    if constexpr (Sh::CREG) {
+      // the transposed inverse: its first stage (radix 6 over the stride-64
+      // classes) on the thread holding the band rows, the rest per owning wave
+      const int w = tid >> 6, lane = tid & 63, c = tid % TK, j0 = tid / TK;
+      cx<T> x[6];
 #pragma unroll
-    for (int q = 0; q < LPS; ++q) {
-      const int idx = tid + q * NT;
-      if (idx < TK * NY) {
-        const int c = idx % TK, j = idx / TK;
-        if (j != NY / 2) {               // zero-pad the band into the 3/2 grid
-          cx<T> v = sv[q];
-          if (d) {
-            const T k = a.kyp[j];
-            v = cx<T>{-k * v.y, k * v.x};
+      for (int q = 0; q < 4; ++q) {
+        cx<T> v = sv[q];
+        if (d) {
+          const T k = a.kyp[j0 + 64 * q];
+          v = cx<T>{-k * v.y, k * v.x};
+        }
+        x[q < 2 ? q : q + 2] = v;
+      }
+      x[2] = x[3] = cx<T>{T(0), T(0)};
+      if (j0 == 0) x[4] = cx<T>{T(0), T(0)};           // the Nyquist row is pad
+      __syncthreads();                                  // twiddle table written
+      fft_first_regs_t<T, NYB>(buf + c * PLB, tw_src(twb), j0, x);
+      __syncthreads();
+      fft_run_t<T, NYB, TK / (NT / 64), 64, n_stages(NYB) - 2, true, PLB>(buf + w * (TK / (NT / 64)) * PLB,
+                                                                        tw_src(twb), lane);
+    } else {
+#pragma unroll
+      for (int q = 0; q < LPS; ++q) {
+        const int idx = tid + q * NT;
+        if (idx < TK * NY) {
+          const int c = idx % TK, j = idx / TK;
+          if (j != NY / 2) {               // zero-pad the band into the 3/2 grid
+            cx<T> v = sv[q];
+            if (d) {
+              const T k = a.kyp[j];
+              v = cx<T>{-k * v.y, k * v.x};
+            }
+            buf[c * PLB + pidx(j < NY / 2 ? j : j - NY + NYB)] = v;
           }
-          buf[c * PLB + pidx(j < NY / 2 ? j : j - NY + NYB)] = v;
         }
       }
+      __syncthreads();                     // the zero pad NY/2 .. NYB-NY/2 is not read (BAND)
+      fft_waves<T, NYB, true, TK, NT / 64, PLB, NY / 2>(buf, twb, tid);
     }
-    __syncthreads();                     // the zero pad NY/2 .. NYB-NY/2 is not read (BAND)
-    fft_waves<T, NYB, true, TK, NT / 64, PLB, NY / 2>(buf, twb, tid);
     __syncthreads();
     cx<T>* od = o + (size_t)d * a.bstride;
 #pragma unroll
@@ -2201,16 +2245,39 @@ __global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
   __shared__ cx<T> tw[NY];
   load_twiddles(tw, a.twNY, NY, tid, NT);
   const cx<T>* src = a.U[f] + (size_t)z * NY * MX + k0;
-  for (int i = tid; i < TK * NY; i += NT) {
-    const int c = i % TK, j = i / TK;
-    cx<T> v = src[(size_t)j * MX + c];
-    if (j == NY / 2) v = cx<T>{0, 0};
-    const T k = a.kyp[j];
-    buf[c * PL + pidx(j)] = v;
-    buf[(TK + c) * PL + pidx(j)] = cx<T>{-k * v.y, k * v.x};
+  if constexpr (Sh::CREG && TK == Sh::TKC) {
+    // transposed inverses: the first stage (radix 4 over the classes j0 + 64 q,
+    // this thread's band rows) of the value and of d/dy here, the rest per
+    // owning wave (two transforms each)
+    const int c = tid % TK, j0 = tid / TK, w = tid >> 6, lane = tid & 63;
+    cx<T> x[4], xd[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = j0 + 64 * q;
+      cx<T> v = src[(size_t)j * MX + c];
+      if (j == NY / 2) v = cx<T>{0, 0};
+      const T k = a.kyp[j];
+      x[q] = v;
+      xd[q] = cx<T>{-k * v.y, k * v.x};
+    }
+    __syncthreads();                                  // twiddle table written
+    fft_first_regs_t<T, NY>(buf + c * PL, tw_src(tw), j0, x);
+    fft_first_regs_t<T, NY>(buf + (TK + c) * PL, tw_src(tw), j0, xd);
+    __syncthreads();
+    fft_run_t<T, NY, 2 * TK / (NT / 64), 64, n_stages(NY) - 2, true, PL>(buf + w * (2 * TK / (NT / 64)) * PL,
+                                                                        tw_src(tw), lane);
+  } else {
+    for (int i = tid; i < TK * NY; i += NT) {
+      const int c = i % TK, j = i / TK;
+      cx<T> v = src[(size_t)j * MX + c];
+      if (j == NY / 2) v = cx<T>{0, 0};
+      const T k = a.kyp[j];
+      buf[c * PL + pidx(j)] = v;
+      buf[(TK + c) * PL + pidx(j)] = cx<T>{-k * v.y, k * v.x};
+    }
+    __syncthreads();
+    fft_waves<T, NY, true, 2 * TK, NT / 64, PL>(buf, tw, tid);
   }
-  __syncthreads();
-  fft_waves<T, NY, true, 2 * TK, NT / 64, PL>(buf, tw, tid);
   __syncthreads();
   const size_t off = (size_t)z * NY * MX + k0;
   cx<T>* o0 = a.S + (size_t)f * a.sstride + off;
