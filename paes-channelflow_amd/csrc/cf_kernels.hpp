@@ -797,6 +797,33 @@ __device__ __forceinline__ void row_prefetch(RowPre<T, MX, TPW>& P, const cx<T>*
   }
 }
 
+// Two levels' synthesis of the lean sweep in one round on all six wavefronts:
+// wave w runs pair w % 3 of the level its prefetched half spectra belong to
+// (the round's first level for waves 0-2, the second for waves 3-5) into slot
+// w (w < 3) or w + 3, so the second level's rows stay in slots 6..8 until the
+// next iteration reads them.  The first member of pair q takes an i kx factor
+// where da[q] says so.  Ends with a workgroup barrier.
+template <typename T, int L, int MX, class TWA>
+__device__ __forceinline__ void synth_two(cx<T>* buf, const RowPre<T, MX, 1>& P, const bool* da,
+                                          const T* __restrict__ kxp, const TWA& tw, int tid) {
+  static_assert(synth_in_regs<T, L, MX>(), "register first stage");
+  const int w = tid >> 6, lane = tid & 63;
+  const int q = w < 3 ? w : w - 3;
+  const bool dqa = pick<3>(da, q, false);
+  cx<T> av[1][MX / 64], bv[1][MX / 64];
+#pragma unroll
+  for (int kk = 0; kk < MX / 64; ++kk) {
+    const int k = lane + 64 * kk;
+    cx<T> x = P.a[0][kk];
+    const T kx = kxp[k];
+    if (dqa) x = cx<T>{-kx * x.y, kx * x.x};
+    av[0][kk] = x;
+    bv[0][kk] = P.b[0][kk];
+  }
+  synth_regs<T, L, MX, 1, padlen(L)>(buf + (w < 3 ? w : w + 3) * padlen(L), av, bv, tw_src(tw), lane);
+  __syncthreads();
+}
+
 // synth_pairs (wave-owned transforms of length L) from prefetched half
 // spectra: the same spectrum, transform and result layout.
 template <typename T, int L, int MX, int TPW, int NP, class TWA>
@@ -1160,7 +1187,10 @@ k_rows(RowArgs<T> a) {
   constexpr bool WV = F::WAVE && (NX <= 256 || !(LEAN && sizeof(T) == 4));
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
   constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
-  __shared__ cx<T> buf[6 * padlen(NXB)];
+  // The lean prefetching sweep synthesises two levels per round (synth_two):
+  // three more slots hold the second level's rows.
+  constexpr bool TWO = LEAN && WV && rows_prefetch<T, NX>() && synth_in_regs<T, NXB, MX>();
+  __shared__ cx<T> buf[(TWO ? 9 : 6) * padlen(NXB)];
   __shared__ cx<T> tw[NXB];
   __shared__ double red[RedSize<NT, 7>::N];
   const int tid = threadIdx.x;
@@ -1268,7 +1298,24 @@ k_rows(RowArgs<T> a) {
       sa[2] = sa[5];
       sb[2] = sb[5];
     }
-    row_prefetch<T, MX, F::TPW, NPS>(P, sa, sb, tid);
+    if constexpr (TWO) {                            // and those of level iz + 1 as pairs 3..5
+      const cx<T>* ta[6];
+      const cx<T>* tb[6];
+      if (iz + 1 < z1) {
+        level_srcs(iz + 1, ta, tb);
+        ta[2] = ta[5];
+        tb[2] = tb[5];
+      } else {
+        for (int q = 0; q < 3; ++q) ta[q] = tb[q] = nullptr;
+      }
+      for (int q = 0; q < 3; ++q) {
+        sa[3 + q] = ta[q];
+        sb[3 + q] = tb[q];
+      }
+      row_prefetch<T, MX, F::TPW, 6>(P, sa, sb, tid);
+    } else {
+      row_prefetch<T, MX, F::TPW, NPS>(P, sa, sb, tid);
+    }
   };
   if constexpr (PRE) prefetch(z0);
   for (int iz = z0; iz < z1; ++iz) {
@@ -1277,7 +1324,22 @@ k_rows(RowArgs<T> a) {
       T ra[3], rb[3];
       const bool da[3] = {false, false, true};
       const bool db[3] = {false, false, false};
-      if constexpr (PRE) {
+      if constexpr (TWO) {
+        // a synthesis round every other level; the second level's rows wait
+        // in slots 6..8
+        const bool first = ((iz - z0) & 1) == 0;
+        if (first) {
+          synth_two<T, NXB, MX>(buf, P, da, a.kxp, twx, tid);
+          if (iz + 2 < z1) prefetch(iz + 2);
+        }
+        const int base = first ? 0 : 6;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const cx<T> v = buf[(base + q) * padlen(NXB) + pidx(tid)];
+          ra[q] = v.x;
+          rb[q] = v.y;
+        }
+      } else if constexpr (PRE) {
         synth_pre<T, NXB, MX, F::TPW, 3>(buf, P, da, db, ra, rb, a.kxp, twx, tid);
         if (iz + 1 < z1) prefetch(iz + 1);
       } else {
