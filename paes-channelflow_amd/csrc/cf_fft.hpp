@@ -315,11 +315,11 @@ __device__ __forceinline__ void fft_run_src(cx<T>* a, const TWS& tws, int tid) {
   }
 }
 
-// The last stage of one wave-owned transform of length N whose last stage has
-// one butterfly per lane (N = 64 R): lane j reads x[j + 64 r], r < R, applies
-// its twiddles and the radix-R butterfly, and keeps the outputs X[j + 64 r] in
-// v[r] instead of storing them (the stages before it ran with SE =
-// n_stages(N) - 1).
+// Butterfly j (< N / R) of the last stage of a transform of length N: reads
+// x[j + (N/R) r], r < R, applies its twiddles and the radix-R butterfly, and
+// keeps the outputs X[j + (N/R) r] in v[r] instead of storing them (the stages
+// before it ran with SE = n_stages(N) - 1).  A wave-owned transform with
+// N = 64 R has one such butterfly per lane, j = lane.
 template <typename T, int N>
 constexpr int last_radix() { return radix_of(N, n_stages(N) - 1); }
 template <typename T, int N, bool INV, class TWS>
@@ -327,7 +327,7 @@ __device__ __forceinline__ void fft_last_regs(const cx<T>* a, const TWS& tws, in
                                               cx<T> (&v)[last_radix<T, N>()]) {
   constexpr int S = n_stages(N) - 1;
   constexpr int R = radix_of(N, S), NS = span_before(N, S), NBF = N / R, STEP = N / (NS * R);
-  static_assert(NBF == 64 && NS == 64, "one butterfly per lane");
+  static_assert(NBF == NS, "the last stage");
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = a[pidx(lane + r * NBF)];
 #pragma unroll
@@ -396,16 +396,17 @@ __device__ __forceinline__ void fft_run_t(cx<T>* a, const TWS& tws, int tid) {
   if constexpr (S > 0) fft_run_t<T, N, NBT, NT, S - 1, WAVE, PLD, TWS>(a, tws, tid);
 }
 
-// The first transposed stage of one wave-owned inverse transform of length
-// N = 64 R from registers: lane j holds x[j + 64 r] in v[r]; inverse
-// butterfly, conjugate twiddles, outputs stored at a[j + 64 r].  The caller
-// orders the stores (wave_sync) before fft_run_t of stages n_stages(N)-2 .. 0.
+// Butterfly j (< N / R) of the first transposed stage of an inverse transform
+// of length N from registers: v[r] holds x[j + (N/R) r]; inverse butterfly,
+// conjugate twiddles, outputs stored at a[j + (N/R) r].  The caller orders the
+// stores (wave_sync or a barrier) before fft_run_t of stages n_stages(N)-2 ..
+// 0.  A wave-owned transform with N = 64 R has one such butterfly per lane.
 template <typename T, int N, class TWS>
 __device__ __forceinline__ void fft_first_regs_t(cx<T>* a, const TWS& tws, int lane,
                                                  cx<T> (&v)[last_radix<T, N>()]) {
   constexpr int S = n_stages(N) - 1;
   constexpr int R = radix_of(N, S), NS = span_before(N, S), NBF = N / R, STEP = N / (NS * R);
-  static_assert(NBF == 64 && NS == 64, "one butterfly per lane");
+  static_assert(NBF == NS, "the last forward stage");
   butterfly<T, R, true>(v);
 #pragma unroll
   for (int r = 1; r < R; ++r) v[r] = cmulc(v[r], tws.template get<S, STEP>(0, r, lane));

@@ -669,6 +669,36 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
     wave_sync();
     if (w * F::TPW < NP)
       fft_run<T, NXB, true, F::TPW, 64, 0, true, padlen(NXB), F::MX>(buf + w * F::TPW * padlen(NXB), tw, lane);
+  } else if constexpr (NP * F::MX == F::NT && last_radix<T, NXB>() == 3) {
+    // workgroup-shared transposed inverses, one first-stage butterfly (radix 3
+    // over the stride-MX classes) per thread: thread (q, j) needs Z_q[j] (mode
+    // j), the pad Z_q[MX + j] and Z_q[2 MX + j], the mirror of mode MX - j
+    // (zero at j = 0, the pad edge), loaded from the half spectra directly
+    constexpr int MX = F::MX;
+    const int q = tid / MX, j = tid - q * MX;
+    const cx<T>* pa = pick<NP>(sa, q, (const cx<T>*)nullptr);
+    const cx<T>* pb = pick<NP>(sb, q, (const cx<T>*)nullptr);
+    const bool dqa = pick<NP>(da, q, false), dqb = pick<NP>(db, q, false);
+    const int m = j == 0 ? 0 : MX - j;
+    cx<T> a0 = pa ? pa[j] : cx<T>{T(0), T(0)}, b0 = pb ? pb[j] : cx<T>{T(0), T(0)};
+    cx<T> am = pa ? pa[m] : cx<T>{T(0), T(0)}, bm = pb ? pb[m] : cx<T>{T(0), T(0)};
+    const T kx0 = kxp[j], kxm = kxp[m];
+    if (dqa) {
+      a0 = cx<T>{-kx0 * a0.y, kx0 * a0.x};
+      am = cx<T>{-kxm * am.y, kxm * am.x};
+    }
+    if (dqb) {
+      b0 = cx<T>{-kx0 * b0.y, kx0 * b0.x};
+      bm = cx<T>{-kxm * bm.y, kxm * bm.x};
+    }
+    cx<T> v[3];
+    // c2r ignores the imaginary part of the DC term
+    v[0] = j == 0 ? cx<T>{a0.x, b0.x} : cx<T>{a0.x - b0.y, a0.y + b0.x};
+    v[1] = cx<T>{T(0), T(0)};
+    v[2] = j == 0 ? cx<T>{T(0), T(0)} : cx<T>{am.x + bm.y, bm.x - am.y};
+    fft_first_regs_t<T, NXB>(buf + q * padlen(NXB), tw_src(tw), j, v);
+    __syncthreads();
+    fft_run_t<T, NXB, NP, F::NT, n_stages(NXB) - 2, false, padlen(NXB)>(buf, tw_src(tw), tid);
   } else {
 #pragma unroll
     for (int q = 0; q < NP; ++q)
@@ -759,6 +789,31 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
       }
     }
     // the next synth_pairs only rewrites this wave's own transforms
+  } else if constexpr (NP * (MX / 2) == F::NT && last_radix<T, NXB>() == 3 && MX % 2 == 0) {
+    // workgroup-shared transforms whose last stage (radix 3, MX butterflies
+    // j, outputs X[j + MX r]) runs in registers: thread (q, jj) takes
+    // butterflies jj and its mirror MX - jj (MX/2 for jj = 0), which hold
+    // both X[k] and X[NXB - k] for k = jj and k = MX - jj (X[NXB - k] =
+    // X[2 MX + (MX - k)])
+    static_assert(!std::is_class<TWA>::value, "workgroup-shared transforms read the LDS table");
+    fft_run_src<T, NXB, false, NP, F::NT, 0, false, padlen(NXB), MX, n_stages(NXB) - 1>(buf, tw_src(tw), tid);
+    __syncthreads();
+    const int q = tid / (MX / 2), jj = tid - q * (MX / 2);
+    const int j2 = jj == 0 ? MX / 2 : MX - jj;
+    cx<T> x1[3], x2[3];
+    fft_last_regs<T, NXB, false>(buf + q * padlen(NXB), tw_src(tw), jj, x1);
+    fft_last_regs<T, NXB, false>(buf + q * padlen(NXB), tw_src(tw), j2, x2);
+    auto put = [&](int k, cx<T> zk, cx<T> zn) {
+      emit(q, k, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s}, cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
+    };
+    if (jj == 0) {
+      put(0, x1[0], x1[0]);
+      put(MX / 2, x2[0], x2[2]);
+    } else {
+      put(jj, x1[0], x2[2]);
+      put(j2, x2[0], x1[2]);
+    }
+    __syncthreads();
   } else {
     static_assert(!std::is_class<TWA>::value, "workgroup-shared transforms read the LDS table");
     fft_run<T, NXB, false, NP, F::NT, 0, false, padlen(NXB), MX>(buf, tw, tid);
