@@ -33,6 +33,10 @@
 
 namespace cf {
 
+// The wavefront index of thread tid as a wave-uniform (scalar) value, so that
+// per-wave choices (which transform, which source row) stay in SGPRs.
+__device__ __forceinline__ int wave_of(int tid) { return __builtin_amdgcn_readfirstlane(tid >> 6); }
+
 // ---------------------------------------------------------------------------
 // Compile-time shape helpers
 // ---------------------------------------------------------------------------
@@ -640,7 +644,7 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   constexpr int NXB = F::NXB;
   static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
   if constexpr (WV && F::TPW == 1 && synth_in_regs<T, NXB, F::MX>()) {
-    const int w = tid >> 6, lane = tid & 63;
+    const int w = wave_of(tid), lane = tid & 63;
     if (w < NP) {
       const cx<T>* pa = pick<NP>(sa, w, (const cx<T>*)nullptr);
       const cx<T>* pb = pick<NP>(sb, w, (const cx<T>*)nullptr);
@@ -660,7 +664,7 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
       synth_regs<T, NXB, F::MX, 1, padlen(NXB)>(buf + w * padlen(NXB), av, bv, tw_src(tw), lane);
     }
   } else if constexpr (WV) {
-    const int w = tid >> 6, lane = tid & 63;
+    const int w = wave_of(tid), lane = tid & 63;
 #pragma unroll
     for (int qq = 0; qq < F::TPW; ++qq) {
       const int q = w * F::TPW + qq;
@@ -767,7 +771,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     // coefficient X[NXB - k] of k = j + 64 kk is X[64 (RL-1-kk) + 64 - j], held
     // by lane 64 - j (k = 64 kk of lane 0: lane 0's own X[64 (RL-kk)], and
     // X[0] for k = 0)
-    const int w = tid >> 6, lane = tid & 63;
+    const int w = wave_of(tid), lane = tid & 63;
     if (w < NP) {
       cx<T>* z = buf + w * padlen(NXB);
       fft_run_src<T, NXB, false, 1, 64, 0, true, padlen(NXB), MX, n_stages(NXB) - 1>(z, tw_src(tw), lane);
@@ -777,7 +781,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     }
     // the next synth_pairs only rewrites this wave's own transforms
   } else if constexpr (WV) {
-    const int w = tid >> 6, lane = tid & 63;
+    const int w = wave_of(tid), lane = tid & 63;
     if (w * F::TPW < NP) {
       fft_run_src<T, NXB, false, F::TPW, 64, 0, true, padlen(NXB), MX>(buf + w * F::TPW * padlen(NXB), tw_src(tw),
                                                                      lane);
@@ -837,7 +841,7 @@ struct RowPre {
 template <typename T, int MX, int TPW, int NP>
 __device__ __forceinline__ void row_prefetch(RowPre<T, MX, TPW>& P, const cx<T>* const* sa,
                                              const cx<T>* const* sb, int tid) {
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
 #pragma unroll
   for (int qq = 0; qq < TPW; ++qq) {
     const int q = w * TPW + qq;
@@ -862,7 +866,7 @@ template <typename T, int L, int MX, class TWA>
 __device__ __forceinline__ void synth_two(cx<T>* buf, const RowPre<T, MX, 1>& P, const bool* da,
                                           const T* __restrict__ kxp, const TWA& tw, int tid) {
   static_assert(synth_in_regs<T, L, MX>(), "register first stage");
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
   const int q = w < 3 ? w : w - 3;
   const bool dqa = pick<3>(da, q, false);
   cx<T> av[1][MX / 64], bv[1][MX / 64];
@@ -885,7 +889,7 @@ template <typename T, int L, int MX, int TPW, int NP, class TWA>
 __device__ __forceinline__ void synth_pre(cx<T>* buf, const RowPre<T, MX, TPW>& P, const bool* da,
                                           const bool* db, T* ra, T* rb, const T* __restrict__ kxp,
                                           const TWA& tw, int tid) {
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
   if constexpr (TPW == 1 && synth_in_regs<T, L, MX>()) {
     if (w < NP) {
       const bool dqa = pick<NP>(da, w, false), dqb = pick<NP>(db, w, false);
@@ -1931,7 +1935,7 @@ __device__ __forceinline__ void synth_small(cx<T>* buf, const cx<T>* const* sa, 
                                             const cx<T>* const* sb, const bool* db, T* ra, T* rb,
                                             const T* __restrict__ kxp, const cx<T>* tw, int tid) {
   using F = SmallRowFFT<T, NX>;
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
   if constexpr (synth_in_regs<T, NX, F::MX>()) {
     if (w * F::TPW < NP) {
       cx<T> av[F::TPW][F::MX / 64], bv[F::TPW][F::MX / 64];
@@ -1988,7 +1992,7 @@ template <typename T, int NX, int NP>
 __device__ __forceinline__ void small_prefetch(SmallPre<T, NX>& P, const cx<T>* const* sa,
                                                const cx<T>* const* sb, int tid) {
   using F = SmallRowFFT<T, NX>;
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
 #pragma unroll
   for (int qq = 0; qq < F::TPW; ++qq) {
     const int q = w * F::TPW + qq;
@@ -2010,7 +2014,7 @@ __device__ __forceinline__ void synth_small_pre(cx<T>* buf, const SmallPre<T, NX
                                                 const cx<T>* tw, int tid) {
   using F = SmallRowFFT<T, NX>;
   constexpr int L = NX, MX = F::MX;
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
   if constexpr (synth_in_regs<T, L, MX>()) {
     if (w * F::TPW < NP) {
       cx<T> av[F::TPW][MX / 64], bv[F::TPW][MX / 64];
@@ -2088,7 +2092,7 @@ __device__ __forceinline__ void anal_small(cx<T>* buf, const T* xa, const T* xb,
 #pragma unroll
   for (int q = 0; q < NP; ++q) buf[q * padlen(NX) + pidx(tid)] = cx<T>{xa[q], xb[q]};
   __syncthreads();
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = wave_of(tid), lane = tid & 63;
   constexpr int RL = last_radix<T, NX>();
   if constexpr (TPWA == 1 && NX == 64 * RL && MX % 64 == 0) {
     // the last stage stays in registers, as in anal_pairs
@@ -2366,7 +2370,7 @@ __global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
     // transposed inverses: the first stage (radix 4 over the classes j0 + 64 q,
     // this thread's band rows) of the value and of d/dy here, the rest per
     // owning wave (two transforms each)
-    const int c = tid % TK, j0 = tid / TK, w = tid >> 6, lane = tid & 63;
+    const int c = tid % TK, j0 = tid / TK, w = wave_of(tid), lane = tid & 63;
     cx<T> x[4], xd[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
