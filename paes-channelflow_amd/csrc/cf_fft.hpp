@@ -191,6 +191,7 @@ __device__ __forceinline__ void fft_sync() {
 // values this thread's butterflies need, held in registers (RegTw).
 template <typename T>
 struct TwLds {
+  static constexpr bool per_lane = false;    // any butterfly's twiddles
   const cx<T>* __restrict__ tw;
   template <int S, int STEP>
   __device__ __forceinline__ cx<T> get(int, int r, int k) const { return tw[r * k * STEP]; }
@@ -202,6 +203,7 @@ struct TwLds {
 // level, so a row kernel loads them once instead of once per stage.
 template <typename T, int N, int NBT, int NT>
 struct RegTw {
+  static constexpr bool per_lane = true;     // only this thread's own butterflies
   static constexpr int NSTG = n_stages(N);
   static constexpr int PERMAX = (NBT * (N / 2) + NT - 1) / NT;
   cx<T> w[NSTG][PERMAX][8];

@@ -36,6 +36,14 @@ namespace cf {
 // The wavefront index of thread tid as a wave-uniform (scalar) value, so that
 // per-wave choices (which transform, which source row) stay in SGPRs.
 __device__ __forceinline__ int wave_of(int tid) { return __builtin_amdgcn_readfirstlane(tid >> 6); }
+// The same for the row transforms of length NXB, except FP64 rows of 768
+// points: there the scalar index raises the register demand past the 168 of
+// three waves per SIMD, and more of the transform spills.
+template <typename T, int NXB>
+__device__ __forceinline__ int row_wave(int tid) {
+  if constexpr (sizeof(T) == 8 && NXB > 384) return tid >> 6;
+  else return wave_of(tid);
+}
 
 // ---------------------------------------------------------------------------
 // Compile-time shape helpers
@@ -580,12 +588,14 @@ __device__ __forceinline__ V pick(const V* arr, int q, V dflt) {
 }
 
 // Whether the synthesis of a wave-owned length-L transform of half spectra
-// k < MX can start from registers: L = 64 R for its first transposed stage
-// (one butterfly per lane), and MX a multiple of 64 so that every stride
-// class j + 64 r is all data, all pad or all mirrored (lane 0 aside).
+// k < MX can start from registers: the first transposed stage (the forward
+// transform's last, radix R over stride L / R) has a whole number of
+// butterflies per lane (L / R a multiple of 64), and MX is a multiple of 64 so
+// that every 64-element block of its inputs is all data, all pad or all
+// mirrored (lane 0 aside).
 template <typename T, int L, int MX>
 constexpr bool synth_in_regs() {
-  return L == 64 * last_radix<T, L>() && MX % 64 == 0 && 2 * MX <= L;
+  return (L / last_radix<T, L>()) % 64 == 0 && MX % 64 == 0 && 2 * MX <= L;
 }
 
 // NB wave-owned inverse transforms of length L (PLD elements apart) whose
@@ -593,39 +603,48 @@ constexpr bool synth_in_regs() {
 // holds modes k = j + 64 kk < MX of transform t in av[t][kk], bv[t][kk] (i kx
 // factors applied).  Z[k] = A[k] + i B[k] for k < MX, Z[L - k] = conj(A[k]) +
 // i conj(B[k]), zero in between (the 3/2-grid pad, or the Nyquist mode of the
-// small grid).  The first transposed stage reads Z[j + 64 r] from registers:
-// classes r below MX are the lane's own modes, classes above L - MX the
-// mirrors of lane 64 - j's (lane 0: its own, or zero at k = L - MX), the rest
-// zero.  The inverse DFTs of the Z are left in z.
+// small grid).  The first transposed stage (radix R over stride NBF = L / R)
+// reads Z[b + NBF r] for its butterflies b = j + 64 i from registers: a
+// 64-block of k below MX is the lane's own modes, one above L - MX the mirrors
+// of lane 64 - j's (lane 0: its own, or zero at k = L - MX), the rest zero.
+// The inverse DFTs of the Z are left in z.
 template <typename T, int L, int MX, int NB, int PLD, class TWS>
 __device__ __forceinline__ void synth_regs(cx<T>* z, const cx<T> (&av)[NB][MX / 64],
                                            const cx<T> (&bv)[NB][MX / 64], const TWS& tws, int lane) {
-  constexpr int KPL = MX / 64, R = last_radix<T, L>();
-  static_assert(synth_in_regs<T, L, MX>(), "one butterfly per lane over whole stride classes");
+  constexpr int KPL = MX / 64, R = last_radix<T, L>(), NBF = L / R, BPL = NBF / 64;
+  static_assert(synth_in_regs<T, L, MX>(), "whole butterflies per lane over whole 64-blocks");
+  static_assert(BPL == 1 || !TWS::per_lane, "register twiddles hold one butterfly per lane");
   const int src = (64 - lane) & 63;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    cx<T> v[R], mir[KPL];
+    cx<T> zo[KPL], mir[KPL];
 #pragma unroll
     for (int kk = 0; kk < KPL; ++kk) {
       const cx<T> a = av[t][kk], b = bv[t][kk];
       // c2r ignores the imaginary part of the DC term
-      v[kk] = (kk == 0 && lane == 0) ? cx<T>{a.x, b.x} : cx<T>{a.x - b.y, a.y + b.x};
+      zo[kk] = (kk == 0 && lane == 0) ? cx<T>{a.x, b.x} : cx<T>{a.x - b.y, a.y + b.x};
       mir[kk] = cx<T>{a.x + b.y, b.x - a.y};
     }
 #pragma unroll
-    for (int r = KPL; r < R; ++r) {
-      if (64 * r + 63 <= L - MX) {
-        v[r] = cx<T>{T(0), T(0)};
-      } else {
-        const int m0 = (L - 64 * r) / 64;           // lane j >= 1: mode 64 (m0 - 1) + 64 - j
-        const cx<T> x = mir[m0 - 1];
-        const cx<T> m = {__shfl(x.x, src, 64), __shfl(x.y, src, 64)};
-        const cx<T> own = (64 * m0 < MX) ? mir[m0 < KPL ? m0 : 0] : cx<T>{T(0), T(0)};
-        v[r] = lane != 0 ? m : own;
+    for (int i = 0; i < BPL; ++i) {
+      cx<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int k0 = 64 * i + NBF * r;              // lane j: k = k0 + j
+        if (k0 + 63 < MX) {
+          v[r] = zo[k0 / 64];
+        } else if (k0 + 63 <= L - MX) {
+          v[r] = cx<T>{T(0), T(0)};
+        } else {
+          const int m0 = (L - k0) / 64;               // lane j >= 1: mode 64 (m0 - 1) + 64 - j
+          const cx<T> x = mir[m0 - 1];
+          const cx<T> m = {__shfl(x.x, src, 64), __shfl(x.y, src, 64)};
+          const cx<T> own = (64 * m0 < MX) ? mir[m0 < KPL ? m0 : 0] : cx<T>{T(0), T(0)};
+          v[r] = lane != 0 ? m : own;
+        }
       }
+      fft_first_regs_t<T, L>(z + t * PLD, tws, lane + 64 * i, v);
     }
-    fft_first_regs_t<T, L>(z + t * PLD, tws, lane, v);
   }
   wave_sync();
   fft_run_t<T, L, NB, 64, n_stages(L) - 2, true, PLD>(z, tws, lane);
@@ -643,7 +662,10 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   using F = RowFFT<T, NX>;
   constexpr int NXB = F::NXB;
   static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
-  if constexpr (WV && F::TPW == 1 && synth_in_regs<T, NXB, F::MX>()) {
+  // (FP64 rows of 768 points keep the spectrum fill: their first transposed
+  // stage from registers spills more of the transform)
+  if constexpr (WV && F::TPW == 1 && synth_in_regs<T, NXB, F::MX>() &&
+                (sizeof(T) == 4 || NXB / last_radix<T, NXB>() == 64)) {
     const int w = wave_of(tid), lane = tid & 63;
     if (w < NP) {
       const cx<T>* pa = pick<NP>(sa, w, (const cx<T>*)nullptr);
@@ -664,7 +686,7 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
       synth_regs<T, NXB, F::MX, 1, padlen(NXB)>(buf + w * padlen(NXB), av, bv, tw_src(tw), lane);
     }
   } else if constexpr (WV) {
-    const int w = wave_of(tid), lane = tid & 63;
+    const int w = row_wave<T, NXB>(tid), lane = tid & 63;
 #pragma unroll
     for (int qq = 0; qq < F::TPW; ++qq) {
       const int q = w * F::TPW + qq;
@@ -781,7 +803,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     }
     // the next synth_pairs only rewrites this wave's own transforms
   } else if constexpr (WV) {
-    const int w = wave_of(tid), lane = tid & 63;
+    const int w = row_wave<T, NXB>(tid), lane = tid & 63;
     if (w * F::TPW < NP) {
       fft_run_src<T, NXB, false, F::TPW, 64, 0, true, padlen(NXB), MX>(buf + w * F::TPW * padlen(NXB), tw_src(tw),
                                                                      lane);
