@@ -99,8 +99,8 @@ MEASURED_TRAFFIC = {
     # profiles/k_rows_mlp_counters_256x256x128.md
     ("k_rows_mlp", 256, 128, "fp32"): round((2 * 202212.8 + 392256.0) * 1024),
     # profiles/learned_small_grid_counters_256x256x128.md
-    ("k_sgs_rows", 256, 128, "fp32"): round((2 * 117375.6 + 196736.0) * 1024),
-    ("k_rows_lean", 256, 128, "fp32"): round((2 * 125534.8 + 391680.0) * 1024),
+    ("k_sgs_rows", 256, 128, "fp32"): round((2 * 117372.5 + 196736.0) * 1024),
+    ("k_rows_lean", 256, 128, "fp32"): round((2 * 127828.4 + 414662.1) * 1024),
 }
 
 # SQ_INSTS_VALU per dispatch (wavefront instructions) at 256x256x128 FP32, from
@@ -109,15 +109,15 @@ MEASURED_TRAFFIC = {
 MEASURED_VALU = {
     ("k_rows", 256, 128, "fp32"): 227.93e6,
     ("k_rows_mlp", 256, 128, "fp32"): 738.28e6,
-    ("k_sgs_rows", 256, 128, "fp32"): 294.34e6,
-    ("k_rows_lean", 256, 128, "fp32"): 173.95e6,
+    ("k_sgs_rows", 256, 128, "fp32"): 290.79e6,
+    ("k_rows_lean", 256, 128, "fp32"): 168.49e6,
 }
 # SQ_ACTIVE_INST_VALU per dispatch (4-cycle issue slots the waves spent in
 # VALU instructions; a transcendental takes two) where profiled
 # (profiles/learned_small_grid_counters_256x256x128.md).
 MEASURED_VALU_ACTIVE = {
-    ("k_sgs_rows", 256, 128, "fp32"): 346.64e6,
-    ("k_rows_lean", 256, 128, "fp32"): 175.13e6,
+    ("k_sgs_rows", 256, 128, "fp32"): 343.09e6,
+    ("k_rows_lean", 256, 128, "fp32"): 169.66e6,
 }
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
