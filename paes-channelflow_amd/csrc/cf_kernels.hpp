@@ -422,9 +422,16 @@ __global__ void __launch_bounds__(512) k_col_synth(SynthArgs<T> a) {
   } else {
     load_twiddles(tws, a.twNY, NY, tid, NT);
     const cx<T>* src = a.Ex + (size_t)z * NY * MX + k0;
-    for (int i = tid; i < TK * NY; i += NT) {
-      const int c = i % TK, y = i / TK;
-      buf[c * PLS + pidx(y)] = src[(size_t)y * MX + c];
+    cx<T> ev[LPS];                       // all loads in flight before the first LDS store
+#pragma unroll
+    for (int q = 0; q < LPS; ++q) {
+      const int i = tid + q * NT;
+      if (i < TK * NY) ev[q] = src[(size_t)(i / TK) * MX + i % TK];
+    }
+#pragma unroll
+    for (int q = 0; q < LPS; ++q) {
+      const int i = tid + q * NT;
+      if (i < TK * NY) buf[(i % TK) * PLS + pidx(i / TK)] = ev[q];
     }
     __syncthreads();
     const T s = T(1) / T(NY);
@@ -2459,24 +2466,47 @@ __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
   const size_t off = (size_t)z * NY * MX + k0;
   const cx<T>* g = a.O + (size_t)(2 * cmp) * a.ostride + off;
   const cx<T>* fy = g + a.ostride;
-  for (int i = tid; i < 2 * TK * NY; i += NT) {
-    const int c = i % TK, r = i / TK;
-    const int y = r % NY, d = r / NY;
-    buf[(d * TK + c) * PL + pidx(y)] = (d ? fy : g)[(size_t)y * MX + c];
+  cx<T>* R = a.R[cmp] + off;
+  // every load of the tile, and of the band values this thread updates, is
+  // issued before the first is used: their latencies overlap
+  constexpr int LPT = (2 * TK * NY + NT - 1) / NT, OPT = (TK * NY + NT - 1) / NT;
+  cx<T> lv[LPT], rv[OPT];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const int i = tid + q * NT;
+    if (i < 2 * TK * NY) {
+      const int c = i % TK, r = i / TK;
+      const int y = r % NY, d = r / NY;
+      lv[q] = (d ? fy : g)[(size_t)y * MX + c];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int i = tid + q * NT;
+    if (i < TK * NY) rv[q] = R[(size_t)(i / TK) * MX + i % TK];
+  }
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const int i = tid + q * NT;
+    if (i < 2 * TK * NY) {
+      const int c = i % TK, r = i / TK;
+      const int y = r % NY, d = r / NY;
+      buf[(d * TK + c) * PL + pidx(y)] = lv[q];
+    }
   }
   __syncthreads();
   fft_waves<T, NY, false, 2 * TK, NT / 64, PL>(buf, tw, tid);
   __syncthreads();
   const T s = T(1) / T(NY);
-  cx<T>* R = a.R[cmp] + off;
-  for (int i = tid; i < TK * NY; i += NT) {
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int i = tid + q * NT;
+    if (i >= TK * NY) continue;
     const int c = i % TK, j = i / TK;
     if (j == NY / 2 || (cmp == 2 && j == 0 && k0 + c == 0)) continue;
     const cx<T> gv = buf[c * PL + pidx(j)], fv = buf[(TK + c) * PL + pidx(j)];
     const T k = a.kyp[j];
-    cx<T>& r = R[(size_t)j * MX + c];
-    const cx<T> rv = r;
-    r = cx<T>{rv.x + (gv.x - k * fv.y) * s, rv.y + (gv.y + k * fv.x) * s};
+    R[(size_t)j * MX + c] = cx<T>{rv[q].x + (gv.x - k * fv.y) * s, rv[q].y + (gv.y + k * fv.x) * s};
   }
 }
 
@@ -3149,6 +3179,19 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
   load_twiddles(tw, a.twNX, NX, tid, NT);
   T ea[2], eb[2];
   if (a.mode == 0) {
+    // e and re_old of this thread's points, loaded before the transform so
+    // that their latency overlaps it
+    T eo0[2], eo1[2], ro0[2] = {T(0), T(0)}, ro1[2] = {T(0), T(0)};
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int x = tid + p * NT;
+      eo0[p] = e[(size_t)y0 * NX + x];
+      eo1[p] = e[(size_t)y1 * NX + x];
+      if (a.istage != 1) {
+        ro0[p] = re[(size_t)y0 * NX + x];
+        ro1[p] = re[(size_t)y1 * NX + x];
+      }
+    }
     // synthesise re on rows y0 (real part) and y1 (imaginary part)
     const cx<T>* s0 = a.Tre + (size_t)z * splane + (size_t)y0 * MX;
     const cx<T>* s1 = a.Tre + (size_t)z * splane + (size_t)y1 * MX;
@@ -3172,14 +3215,14 @@ __global__ void __launch_bounds__(NX / 2) k_erows(ERowArgs<T> a) {
       T* pe1 = e + (size_t)y1 * NX + x;
       T* pr0 = re + (size_t)y0 * NX + x;
       T* pr1 = re + (size_t)y1 * NX + x;
-      T e0 = *pe0, e1 = *pe1;
+      T e0 = eo0[p], e1 = eo1[p];
       if (z == nz - 1) {                       // upper: e(nnz) = e(nnz-1) (upperbc.f90:20-24)
         e[(size_t)plane + (size_t)y0 * NX + x] = e0;
         e[(size_t)plane + (size_t)y1 * NX + x] = e1;
       }
       if (a.istage != 1) {
-        e0 = e0 - (T)a.dtab2 * *pr0;
-        e1 = e1 - (T)a.dtab2 * *pr1;
+        e0 = e0 - (T)a.dtab2 * ro0[p];
+        e1 = e1 - (T)a.dtab2 * ro1[p];
       }
       e0 = e0 + (T)a.dtab1 * r.x;
       e1 = e1 + (T)a.dtab1 * r.y;
