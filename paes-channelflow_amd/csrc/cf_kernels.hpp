@@ -2916,26 +2916,37 @@ k_press_part(PressArgs<T> a) {
   const size_t pl = (size_t)modes;
   const bool sub = (a.istage != 1);
 
-  // loads: the chunk's levels, and w, rw of the level below it
+  // loads: the chunk's levels, and w, rw of the level below it.  They are
+  // unconditional, from a valid mode and level (a level past the chunk reads
+  // level nz, an inactive thread mode 0), so that every load of the chunk is
+  // in flight before the first is used; the values outside are zeroed after.
   cx<T> lu[C], lv[C], lw[C], ln0[C], ln1[C], ln2[C], ou[C], ov[C], ow[C];
   const cx<T> zt = {T(0), T(0)};
+  const int mc = m < modes ? m : 0;
+  const int lb = lo > 1 ? (lo - 1 <= nz ? lo - 1 : nz) : 1;
 #pragma unroll
   for (int q = 0; q < C; ++q) {
-    lu[q] = lv[q] = lw[q] = ln0[q] = ln1[q] = ln2[q] = ou[q] = ov[q] = ow[q] = zt;
-    if (live && q < n) {
-      const size_t o = (size_t)(lo + q) * pl + m;
-      lu[q] = a.U[0][o]; lv[q] = a.U[1][o]; lw[q] = a.U[2][o];
-      ln0[q] = a.Rn[0][o]; ln1[q] = a.Rn[1][o]; ln2[q] = a.Rn[2][o];
-      if (sub) { ou[q] = a.Ro[0][o]; ov[q] = a.Ro[1][o]; ow[q] = a.Ro[2][o]; }
+    const size_t o = (size_t)(lo + q <= nz ? lo + q : nz) * pl + mc;
+    lu[q] = a.U[0][o]; lv[q] = a.U[1][o]; lw[q] = a.U[2][o];
+    ln0[q] = a.Rn[0][o]; ln1[q] = a.Rn[1][o]; ln2[q] = a.Rn[2][o];
+  }
+  cx<T> wb = a.U[2][(size_t)lb * pl + mc], nb = a.Rn[2][(size_t)lb * pl + mc], ob = zt;
+  if (sub) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const size_t o = (size_t)(lo + q <= nz ? lo + q : nz) * pl + mc;
+      ou[q] = a.Ro[0][o]; ov[q] = a.Ro[1][o]; ow[q] = a.Ro[2][o];
     }
+    ob = a.Ro[2][(size_t)lb * pl + mc];
+  } else {
+#pragma unroll
+    for (int q = 0; q < C; ++q) ou[q] = ov[q] = ow[q] = zt;
   }
-  cx<T> wb = zt, nb = zt, ob = zt;
-  if (live && lo > 1) {
-    const size_t o = (size_t)(lo - 1) * pl + m;
-    wb = a.U[2][o];
-    nb = a.Rn[2][o];
-    if (sub) ob = a.Ro[2][o];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    if (!(live && q < n)) lu[q] = lv[q] = lw[q] = ln0[q] = ln1[q] = ln2[q] = ou[q] = ov[q] = ow[q] = zt;
   }
+  if (!(live && lo > 1)) wb = nb = ob = zt;
 
   // substep, rhs and the interior elimination (comp1.f90:126-133,
   // comprhs.f90:17-35, tridv.f90:34-43); g's rhs is e_first
