@@ -3403,14 +3403,18 @@ __global__ void __launch_bounds__(256) k_divz_band(const cx<T>* __restrict__ U, 
   const cx<T>* w = W + (size_t)iz * plane;
   const cx<T>* wm = W + (size_t)(iz - 1) * plane;
   double s = 0.0;
+  // w of face nz and of face 0 are read (valid levels) and replaced by zero,
+  // so that the loads carry no branch and several iterations' loads overlap
+  const bool hw = iz < nz, hm = iz > 1;
+#pragma unroll 4
   for (size_t m = (size_t)blockIdx.y * 256 + tid; m < plane; m += (size_t)gridDim.y * 256) {
     const int k = (int)(m % mx), j = (int)(m / mx);
     const double kx = (double)kxp[k], ky = (double)kyp[j];
-    double wr = 0, wi = 0, mr = 0, mi = 0;
-    if (iz < nz) { wr = (double)w[m].x; wi = (double)w[m].y; }
-    if (iz > 1) { mr = (double)wm[m].x; mi = (double)wm[m].y; }
-    const double dr = -kx * (double)u[m].y - ky * (double)v[m].y + (wr - mr) * dzm;
-    const double di = kx * (double)u[m].x + ky * (double)v[m].x + (wi - mi) * dzm;
+    const cx<T> uu = u[m], vv = v[m], ww = w[m], wb = wm[m];
+    const double wr = hw ? (double)ww.x : 0.0, wi = hw ? (double)ww.y : 0.0;
+    const double mr = hm ? (double)wb.x : 0.0, mi = hm ? (double)wb.y : 0.0;
+    const double dr = -kx * (double)uu.y - ky * (double)vv.y + (wr - mr) * dzm;
+    const double di = kx * (double)uu.x + ky * (double)vv.x + (wi - mi) * dzm;
     s += (k == 0 ? 1.0 : 2.0) * (dr * dr + di * di);
   }
   red[tid] = s;

@@ -260,15 +260,18 @@ def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
     net.close()
 
 
-@pytest.mark.parametrize("n", [256, 512])
-def test_fp32_small_grid_stage_close_to_fp64(n):
+@pytest.mark.parametrize("n,nz", [(256, 4), (256, 11), (512, 4)])
+def test_fp32_small_grid_stage_close_to_fp64(n, nz):
     """The FP32 small-grid closure stage (lean 3/2-grid sweep: wave-owned
     transforms at 256^2, workgroup-shared at 512^2) against the FP64 one,
-    which the oracle pins, from the same state.  With a network that every
-    cell rejects no acceptance decision can flip, so the fields differ by FP32
-    rounding alone: within 2e-5 relative L2, as the FP32 TKE step."""
+    which the oracle pins, from the same state.  At 256^2 the FP32 sweep
+    synthesises two levels per round; nnz = 11 gives chunks of 8 and 3
+    levels, so the last round of the second chunk holds one level.  With a
+    network that every cell rejects no acceptance decision can flip, so the
+    fields differ by FP32 rounding alone: within 2e-5 relative L2, as the
+    FP32 TKE step."""
     import channelflow as C
-    pr = O.Params(n, n, 4, dt=0.002)
+    pr = O.Params(n, n, nz, dt=0.002)
     u, v, w, e = O.initial_fields(pr)
     flat, norm = _closure_net("reject")
     net = make_net(flat, norm)
