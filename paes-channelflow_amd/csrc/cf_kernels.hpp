@@ -2533,8 +2533,12 @@ struct AnalArgs {
   int nz;
 };
 
+// FP32 at ny = 512: at most 128 VGPRs, four waves per SIMD, so that the two
+// workgroups the LDS allows share a CU (left alone the compiler takes 148-162
+// and one workgroup per CU: 1.83 -> 1.23 ms per launch at 512^2 x 256).
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__(512) k_col_anal(AnalArgs<T> a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NY == 512 ? 4 : 1)))
+k_col_anal(AnalArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
   static_assert(Sh::NT == 512, "launch bound");
   constexpr int MX = Sh::MX, NYB = Sh::NYB, TK = Sh::TKC, NT = Sh::NT;
