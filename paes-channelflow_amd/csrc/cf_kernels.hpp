@@ -1257,10 +1257,11 @@ __global__ void __launch_bounds__(3 * NX / 2) k_shear(const cx<T>* __restrict__ 
 // divergence, so the velocity gradients are not synthesised: three pairs per
 // level instead of six (u, v; w, e; ex, ey).
 // PRE: the synthesis half spectra are loaded a level ahead.  Measured and
-// tuned for FP32 at nnx = 256, where it fits four waves per SIMD (128 VGPRs,
-// no spill; left alone the compiler takes 133-136 and three waves).  Other
-// shapes keep the direct loads: there the extra registers spill or cost a
-// wave per SIMD.
+// tuned for FP32 at nnx = 256, where it is held at four waves per SIMD (128
+// VGPRs; the lean sweep spills 24 bytes per lane, the full one none; left
+// alone the compiler takes 133-148 and three waves, which measured slower:
+// lean 0.507 -> 0.662 ms).  Other shapes keep the direct loads: there the
+// extra registers spill or cost a wave per SIMD.
 template <typename T, int NX>
 constexpr bool rows_prefetch() { return sizeof(T) == 4 && NX == 256; }
 
