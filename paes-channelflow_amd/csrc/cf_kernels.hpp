@@ -748,6 +748,13 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   }
 }
 
+// Whether a twiddle source holds only this thread's own butterflies (RegTw).
+template <class X>
+constexpr bool tw_per_lane() {
+  if constexpr (std::is_class<X>::value) return X::per_lane;
+  else return false;
+}
+
 // Coefficients k < MX of the two real rows of transform q from the outputs of
 // a wave-owned forward transform of length L = 64 R kept in registers (lane j
 // holds X[j + 64 r] in v[r]), scaled by s, handed to emit(q, k, A, B).  The
@@ -807,6 +814,34 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
       cx<T> v[RL];
       fft_last_regs<T, NXB, false>(z, tw_src(tw), lane, v);
       emit_from_regs<T, NXB, MX>(v, lane, s, w, emit);
+    }
+    // the next synth_pairs only rewrites this wave's own transforms
+  } else if constexpr (WV && F::TPW == 1 && sizeof(T) == 4 && RL == 3 && NXB == 3 * MX && MX % 64 == 0) {
+    // FP32 rows whose last stage is radix 3 over stride MX (768 = 8.8.4.3):
+    // lane j keeps the outputs of its butterflies b = j + 64 i, X[b + MX r],
+    // in registers.  Coefficient k = j + 64 kk is X[k] of butterfly k; its
+    // mirror X[NXB - k] = X[2 MX + (MX - k)] is output 2 of butterfly MX - k
+    // = 64 (KK-1-kk) + 64 - j, held by lane 64 - j (lane 0: its own butterfly
+    // 64 (KK - kk), and X[0] for k = 0).
+    static_assert(!tw_per_lane<TWA>(), "any butterfly's twiddles");
+    constexpr int KK = MX / 64;
+    const int w = wave_of(tid), lane = tid & 63;
+    if (w < NP) {
+      cx<T>* z = buf + w * padlen(NXB);
+      fft_run_src<T, NXB, false, 1, 64, 0, true, padlen(NXB), MX, n_stages(NXB) - 1>(z, tw_src(tw), lane);
+      cx<T> x[KK][3];
+#pragma unroll
+      for (int i = 0; i < KK; ++i) fft_last_regs<T, NXB, false>(z, tw_src(tw), lane + 64 * i, x[i]);
+      const int src = (64 - lane) & 63;
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk) {
+        const cx<T> y = x[KK - 1 - kk][2];
+        const cx<T> m = {__shfl(y.x, src, 64), __shfl(y.y, src, 64)};
+        const cx<T> zk = x[kk][0];
+        const cx<T> zn = lane != 0 ? m : (kk == 0 ? zk : x[KK - kk < KK ? KK - kk : 0][2]);
+        emit(w, lane + 64 * kk, cx<T>{(zk.x + zn.x) * s, (zk.y - zn.y) * s},
+             cx<T>{(zk.y + zn.y) * s, (zn.x - zk.x) * s});
+      }
     }
     // the next synth_pairs only rewrites this wave's own transforms
   } else if constexpr (WV) {
