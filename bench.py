@@ -120,6 +120,13 @@ MEASURED_VALU_ACTIVE = {
     ("k_rows_lean", 256, 128, "fp32"): 169.66e6,
 }
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
+# SQ_INSTS_MFMA per dispatch (v_mfma_f32_32x32x16_bf16, 32 x 32 x 16 x 2 flop
+# each) from the same counter profiles, against the dense BF16 peak.
+MEASURED_MFMA = {
+    ("k_sgs_rows", 256, 128, "fp32"): 5.95e6,
+}
+MFMA_FLOP = 32 * 32 * 16 * 2
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def valu_roofline(kernel, args, ms):
@@ -135,6 +142,18 @@ def valu_roofline(kernel, args, ms):
     if active is not None:
         out["busy_frac"] = active / (ms * 1e-3) / VALU_PEAK
     return out
+
+
+def mfma_roofline(kernel, args, ms):
+    """The dominant kernel's BF16 matrix-core work against the dense BF16
+    peak: SQ_INSTS_MFMA per launch x flop per instruction / launch time.
+    None where not profiled."""
+    n = MEASURED_MFMA.get((kernel, args.nx, args.nz, "fp32"))
+    if n is None:
+        return None
+    achieved = n * MFMA_FLOP / (ms * 1e-3) / 1e12
+    return {"achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / BF16_PEAK_TFLOPS, "mfma_per_launch": n}
 
 
 def parse():
@@ -341,6 +360,7 @@ def main():
             "basis": f"{dom_label}: {BASIS.get(dom_label, 'algorithmic bytes per launch')} "
                      "/ mean launch time from HIP events on the library stream",
             "valu": valu_roofline(dom_label, args, kern[dom]),
+            "mfma": mfma_roofline(dom_label, args, kern[dom]),
         },
         "step_roofline": {
             "achieved": step_achieved,
