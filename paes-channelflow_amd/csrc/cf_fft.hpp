@@ -97,6 +97,19 @@ constexpr int padlen(int n) { return n + n / 8; }
 constexpr int colpad(int n) { return padlen(n) + ((8 - padlen(n) % 32) + 32) % 32; }
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
 
+// Element base + r * S of a padded transform: pidx(base + r S) = pidx(base)
+// + r S + r S / 8 when S is a multiple of 8, and pidx(base) + r when S = 1,
+// base is a multiple of 8 and r < 8.  Written this way the r-dependent part
+// is a constant that the LDS instruction's offset field carries, instead of
+// one address register per r.  The FP64 kernels use it (FP64 k_rows at 256^2
+// 1.27 -> 1.23 ms, 161 -> 135 VGPRs); the FP32 ones measured slower with it
+// (FP32 k_rows 0.638 -> 0.659 ms, the nnx 512 lean sweep 5.77 -> 6.11 ms)
+// and keep pidx of the full index.
+template <typename T, int S, bool BASE8 = false>
+constexpr bool pofs_ok() { return sizeof(T) == 8 && (S % 8 == 0 || (S == 1 && BASE8)); }
+template <int S>
+constexpr int pofs(int r) { return S % 8 == 0 ? r * S + (r * S) / 8 : r * S; }
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void butterfly(cx<T>* v) {
   if constexpr (R == 2) {
@@ -272,6 +285,8 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
           const cx<T> x = src[pidx(p)];
           const bool pad = p >= BAND && p <= N - BAND;
           v[q][r] = cx<T>{pad ? T(0) : x.x, pad ? T(0) : x.y};
+        } else if constexpr (pofs_ok<T, NBF>()) {
+          v[q][r] = (src + pidx(j))[pofs<NBF>(r)];
         } else {
           v[q][r] = src[pidx(j + r * NBF)];
         }
@@ -300,7 +315,8 @@ __device__ __forceinline__ void fft_stage(cx<T>* a, const TWS& tws, int tid) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (ZOUT && r * NS >= BAND && r * NS + NS - 1 <= N - BAND) continue;
-        dst[pidx(base + r * NS)] = v[q][r];
+        if constexpr (pofs_ok<T, NS, R == 8>()) dst[pidx(base) + pofs<NS>(r)] = v[q][r];
+        else dst[pidx(base + r * NS)] = v[q][r];
       }
     }
   }
@@ -331,7 +347,10 @@ __device__ __forceinline__ void fft_last_regs(const cx<T>* a, const TWS& tws, in
   constexpr int R = radix_of(N, S), NS = span_before(N, S), NBF = N / R, STEP = N / (NS * R);
   static_assert(NBF == NS, "the last stage");
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = a[pidx(lane + r * NBF)];
+  for (int r = 0; r < R; ++r) {
+    if constexpr (pofs_ok<T, NBF>()) v[r] = (a + pidx(lane))[pofs<NBF>(r)];
+    else v[r] = a[pidx(lane + r * NBF)];
+  }
 #pragma unroll
   for (int r = 1; r < R; ++r) {
     const cx<T> w = tws.template get<S, STEP>(0, r, lane);
@@ -368,7 +387,10 @@ __device__ __forceinline__ void fft_stage_t(cx<T>* a, const TWS& tws, int tid) {
       const cx<T>* src = a + t * PLD;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[pidx(base + r * NS)];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (pofs_ok<T, NS, R == 8>()) v[q][r] = src[pidx(base) + pofs<NS>(r)];
+        else v[q][r] = src[pidx(base + r * NS)];
+      }
       butterfly<T, R, true>(v[q]);
       if constexpr (NS > 1) {
 #pragma unroll
@@ -385,7 +407,10 @@ __device__ __forceinline__ void fft_stage_t(cx<T>* a, const TWS& tws, int tid) {
       const int j = idx - t * NBF;
       cx<T>* dst = a + t * PLD;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[pidx(j + r * NBF)] = v[q][r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (pofs_ok<T, NBF>()) dst[pidx(j) + pofs<NBF>(r)] = v[q][r];
+        else dst[pidx(j + r * NBF)] = v[q][r];
+      }
     }
   }
   fft_sync<WAVE>();
@@ -413,7 +438,10 @@ __device__ __forceinline__ void fft_first_regs_t(cx<T>* a, const TWS& tws, int l
 #pragma unroll
   for (int r = 1; r < R; ++r) v[r] = cmulc(v[r], tws.template get<S, STEP>(0, r, lane));
 #pragma unroll
-  for (int r = 0; r < R; ++r) a[pidx(lane + r * NBF)] = v[r];
+  for (int r = 0; r < R; ++r) {
+    if constexpr (pofs_ok<T, NBF>()) a[pidx(lane) + pofs<NBF>(r)] = v[r];
+    else a[pidx(lane + r * NBF)] = v[r];
+  }
 }
 
 template <typename T, int N, bool INV, int NBT, int NT, int S = 0, bool WAVE = false, int PLD = padlen(N),
