@@ -661,8 +661,10 @@ __device__ __forceinline__ void synth_regs(cx<T>* z, const cx<T> (&av)[NB][MX / 
 // rows of pair q comes back in ra[q], rb[q].  Ends with a workgroup barrier
 // after which the caller must not write buf until the next anal_pairs.
 // WV: each wavefront owns whole transforms (needs NXB a multiple of 64);
-// otherwise the workgroup shares every transform.
-template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE>
+// otherwise the workgroup shares every transform.  PPT > 1 (NXB / PPT threads,
+// at least NP wavefronts, wave-owned transforms from registers): thread t
+// holds points t + p NXB / PPT, returned in ra[p NP + q], rb[p NP + q].
+template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE, int PPT = 1>
 __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, const bool* da,
                                             const cx<T>* const* sb, const bool* db, T* ra, T* rb,
                                             const T* __restrict__ kxp, const cx<T>* tw, int tid) {
@@ -671,8 +673,10 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
   static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
   // (FP64 rows of 768 points keep the spectrum fill: their first transposed
   // stage from registers spills more of the transform)
-  if constexpr (WV && F::TPW == 1 && synth_in_regs<T, NXB, F::MX>() &&
-                (sizeof(T) == 4 || NXB / last_radix<T, NXB>() == 64)) {
+  constexpr bool REGS = WV && F::TPW == 1 && synth_in_regs<T, NXB, F::MX>() &&
+                        (sizeof(T) == 4 || NXB / last_radix<T, NXB>() == 64);
+  static_assert(PPT == 1 || REGS, "several points per thread: register synthesis only");
+  if constexpr (REGS) {
     const int w = wave_of(tid), lane = tid & 63;
     if (w < NP) {
       const cx<T>* pa = pick<NP>(sa, w, (const cx<T>*)nullptr);
@@ -740,11 +744,22 @@ __device__ __forceinline__ void synth_pairs(cx<T>* buf, const cx<T>* const* sa, 
     fft_run<T, NXB, true, NP, F::NT, 0, false, padlen(NXB), F::MX>(buf, tw, tid);
   }
   __syncthreads();
+  if constexpr (PPT == 1) {
 #pragma unroll
-  for (int q = 0; q < NP; ++q) {
-    const cx<T> v = buf[q * padlen(NXB) + pidx(tid)];
-    ra[q] = v.x;
-    rb[q] = v.y;
+    for (int q = 0; q < NP; ++q) {
+      const cx<T> v = buf[q * padlen(NXB) + pidx(tid)];
+      ra[q] = v.x;
+      rb[q] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const cx<T> v = buf[q * padlen(NXB) + pidx(tid + p * (NXB / PPT))];
+        ra[p * NP + q] = v.x;
+        rb[p * NP + q] = v.y;
+      }
   }
 }
 
@@ -783,8 +798,10 @@ __device__ __forceinline__ void emit_from_regs(const cx<T> (&v)[last_radix<T, L>
 
 // Forward-transform NP real row pairs (this thread holds point tid of each)
 // and hand coefficient k < MX of both members, scaled by 1/NXB, to
-// emit(q, k, A, B).
-template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE, typename Emit, class TWA>
+// emit(q, k, A, B).  PPT > 1 (NXB / PPT threads, at least NP wavefronts,
+// wave-owned transforms with the last stage in registers): thread t holds
+// points t + p NXB / PPT, in xa[p NP + q], xb[p NP + q].
+template <typename T, int NX, int NP, bool WV = RowFFT<T, NX>::WAVE, int PPT = 1, typename Emit, class TWA>
 __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb, const TWA& tw,
                                            int tid, Emit emit) {
   using F = RowFFT<T, NX>;
@@ -792,8 +809,16 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
   static_assert(!WV || F::WAVE, "wave-owned transforms need whole wavefronts");
   const T s = T(0.5) / T(NXB);
   __syncthreads();                       // every thread has read the synthesis results
+  if constexpr (PPT == 1) {
 #pragma unroll
-  for (int q = 0; q < NP; ++q) buf[q * padlen(NXB) + pidx(tid)] = cx<T>{xa[q], xb[q]};
+    for (int q = 0; q < NP; ++q) buf[q * padlen(NXB) + pidx(tid)] = cx<T>{xa[q], xb[q]};
+  } else {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        buf[q * padlen(NXB) + pidx(tid + p * (NXB / PPT))] = cx<T>{xa[p * NP + q], xb[p * NP + q]};
+  }
   __syncthreads();
   auto extract = [&](int q, int k) {
     const cx<T> zk = buf[q * padlen(NXB) + pidx(k)];
@@ -845,6 +870,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     }
     // the next synth_pairs only rewrites this wave's own transforms
   } else if constexpr (WV) {
+    static_assert(PPT == 1, "several points per thread: register last stage only");
     const int w = row_wave<T, NXB>(tid), lane = tid & 63;
     if (w * F::TPW < NP) {
       fft_run_src<T, NXB, false, F::TPW, 64, 0, true, padlen(NXB), MX>(buf + w * F::TPW * padlen(NXB), tw_src(tw),
@@ -857,7 +883,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
       }
     }
     // the next synth_pairs only rewrites this wave's own transforms
-  } else if constexpr (NP * (MX / 2) == F::NT && last_radix<T, NXB>() == 3 && MX % 2 == 0) {
+  } else if constexpr (PPT == 1 && NP * (MX / 2) == F::NT && last_radix<T, NXB>() == 3 && MX % 2 == 0) {
     // workgroup-shared transforms whose last stage (radix 3, MX butterflies
     // j, outputs X[j + MX r]) runs in registers: thread (q, jj) takes
     // butterflies jj and its mirror MX - jj (MX/2 for jj = 0), which hold
@@ -883,6 +909,7 @@ __device__ __forceinline__ void anal_pairs(cx<T>* buf, const T* xa, const T* xb,
     }
     __syncthreads();
   } else {
+    static_assert(PPT == 1, "several points per thread: register last stage only");
     static_assert(!std::is_class<TWA>::value, "workgroup-shared transforms read the LDS table");
     fft_run<T, NXB, false, NP, F::NT, 0, false, padlen(NXB), MX>(buf, tw, tid);
     if (tid < MX) {
@@ -1581,6 +1608,10 @@ k_rows(RowArgs<T> a) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sums[q] += __shfl_xor(sums[q], off, 64);
       }
+      // the second level of a two-level round has no synthesis barrier
+      // between the previous level's reading of the partials and this write
+      if constexpr (TWO)
+        if (mn && ((iz - z0) & 1)) __syncthreads();
       if ((tid & 63) == 0)
         for (int q = 1; q < nq; ++q) red[q * F::NW + (tid >> 6)] = sums[q];
     } else {
@@ -1624,6 +1655,245 @@ k_rows(RowArgs<T> a) {
     wm = w0; w0 = wp;
     em = e0; e0 = ep;
     wxm = wx0; wym = wy0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_rows_lean2: the lean sweep of k_rows<T, NX, true> (the small-grid closure
+// supplies the SGS momentum stresses; the walls and the TKE equation stay
+// here) for FP32 rows of 768 points, two points per thread: six wavefronts
+// per row, each owning whole transforms -- the synthesis pairs of two levels
+// per round (synth_two) and one analysis pair per level -- where k_rows'
+// twelve wavefronts would share every transform or leave half of them idle
+// (5.76 -> 5.54 ms per launch at 512^2 x 256).  Same arguments, grid (NYB,
+// z-chunks) and outputs as k_rows<T, NX, true>.
+// ---------------------------------------------------------------------------
+template <typename T, int NX>
+__global__ void __launch_bounds__(3 * NX / 4) __attribute__((amdgpu_waves_per_eu(3)))
+k_rows_lean2(RowArgs<T> a) {
+  using F = RowFFT<T, NX>;
+  constexpr int MX = F::MX, NXB = F::NXB, NT = NXB / 2, NW = NT / 64;
+  constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
+  static_assert(sizeof(T) == 4 && NT % 64 == 0 && NW == 6 && synth_in_regs<T, NXB, MX>(),
+                "six wave-owned FP32 transforms per level, first stage from registers");
+  // buf: the first level's synthesis rows in slots 0..2, the second's in 6..8,
+  // the analysis pairs in 0..5.  red: two sets of wave partials, alternating
+  // by level (no barrier separates one level's reading of them from the next
+  // level's writing).
+  constexpr int RN = 7 * NW;
+  __shared__ cx<T> buf[9 * padlen(NXB)];
+  __shared__ cx<T> tw[NXB];
+  __shared__ double red[2 * RN];
+  const int tid = threadIdx.x;
+  const int y = blockIdx.x;
+  const Consts& c = a.c;
+  const int nz = c.nz;
+  const int z0 = 1 + blockIdx.y * c.zchunk;
+  int z1 = z0 + c.zchunk;
+  if (z1 > nz + 1) z1 = nz + 1;
+  const size_t plane = (size_t)NYB * MX;
+  const size_t rowoff = (size_t)y * MX;
+  auto row = [&](int fld, int L) -> const cx<T>* {
+    return a.B + (size_t)fld * a.bstride + (size_t)L * plane + rowoff;
+  };
+  auto clampF = [&](int F) { return F <= 0 ? 1 : (F >= nz ? nz - 1 : F); };
+  enum { FU = 0, FUY, FV, FVY, FW, FWY, FE, FEY };
+  load_twiddles(tw, a.twNXB, NXB, tid, NT);
+  __syncthreads();                       // the table is read by every wave's transforms
+
+  const T emin = (T)c.emin;
+  const T alk = (T)c.alk, xkmax = (T)c.xkmax;
+  const T dzm = (T)c.dzm, todzm = (T)c.todzm, dzm2 = (T)c.dzm2, ceps = (T)c.ceps;
+  const bool mn = c.mnout != 0;
+  auto xkk_of = [&](T e) {
+    T v = T(0.1) * alk * sqrt(e);
+    return v < xkmax ? v : xkmax;   // complen.f90:21
+  };
+
+  // window registers of the thread's points tid and tid + NT (levels iz-1,
+  // iz, iz+1 of u, v; faces of w, e)
+  T um[2], u0[2], up[2], vm[2], v0[2], vp[2], wm[2], w0[2], wp[2], em[2], e0[2], ep[2];
+  {
+    // initial window: u,v at z0-1, z0 ; w,e at faces z0-1, z0
+    const cx<T>* sa[4];
+    const cx<T>* sb[4];
+    const bool da[4] = {false, false, false, false};
+    const bool db[4] = {false, false, false, false};
+    T ra[8], rb[8];
+    const int L0 = z0 - 1, L1 = z0;
+    sa[0] = (L0 >= 1 && L0 <= nz) ? row(FU, L0) : nullptr;
+    sb[0] = (L0 >= 1 && L0 <= nz) ? row(FV, L0) : nullptr;
+    sa[1] = row(FU, L1);
+    sb[1] = row(FV, L1);
+    sa[2] = (L0 >= 1 && L0 <= nz - 1) ? row(FW, L0) : nullptr;
+    sb[2] = row(FE, clampF(L0));
+    sa[3] = (L1 >= 1 && L1 <= nz - 1) ? row(FW, L1) : nullptr;
+    sb[3] = row(FE, clampF(L1));
+    synth_pairs<T, NX, 4, true, 2>(buf, sa, da, sb, db, ra, rb, a.kxp, tw, tid);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      um[p] = ra[4 * p]; vm[p] = rb[4 * p];
+      u0[p] = ra[4 * p + 1]; v0[p] = rb[4 * p + 1];
+      wm[p] = ra[4 * p + 2]; w0[p] = ra[4 * p + 3];
+      if (rb[4 * p + 2] < emin) mark_clamp(a.dirty, clampF(L0));
+      if (rb[4 * p + 3] < emin) mark_clamp(a.dirty, clampF(L1));
+      em[p] = rb[4 * p + 2] > emin ? rb[4 * p + 2] : emin;
+      e0[p] = rb[4 * p + 3] > emin ? rb[4 * p + 3] : emin;
+    }
+    __syncthreads();                     // all reads done before waves refill buf
+  }
+
+  // the three synthesis pairs of level iz: (u, v) at iz+1, (w, e) at face
+  // iz+1, (de/dx, de/dy) at face iz -- face nnz-2 at iz = nnz-1, where ex, ey
+  // are not refreshed (derivs2.f90:9,20-21)
+  auto level_srcs = [&](int iz, const cx<T>** sa, const cx<T>** sb) {
+    const int Lp = iz + 1;
+    sa[0] = (Lp <= nz) ? row(FU, Lp) : nullptr;
+    sb[0] = (Lp <= nz) ? row(FV, Lp) : nullptr;
+    sa[1] = (Lp <= nz - 1) ? row(FW, Lp) : nullptr;
+    sb[1] = row(FE, clampF(Lp));
+    const int ie = (iz == nz - 1) ? nz - 2 : iz;
+    sa[2] = (iz <= nz - 1) ? row(FE, ie) : nullptr;
+    sb[2] = (iz <= nz - 1) ? row(FEY, ie) : nullptr;
+  };
+  // the half spectra of a round: level iz's pairs for waves 0..2, level
+  // iz+1's for waves 3..5.  They are loaded at the round's start: held in
+  // registers a round ahead (as the nnx = 256 sweep does) they push 38
+  // registers of the transforms into scratch (measured 6.40 against 5.54 ms
+  // per launch at 512^2 x 256).
+  RowPre<T, MX, 1> P;
+  auto load_round = [&](int iz) {
+    const cx<T>* sa[6];
+    const cx<T>* sb[6];
+    level_srcs(iz, sa, sb);
+    if (iz + 1 < z1) {
+      level_srcs(iz + 1, sa + 3, sb + 3);
+    } else {
+      for (int q = 3; q < 6; ++q) sa[q] = sb[q] = nullptr;
+    }
+    row_prefetch<T, MX, 1, 6>(P, sa, sb, tid);
+  };
+  const bool dsyn[3] = {false, false, true};
+  for (int iz = z0; iz < z1; ++iz) {
+    const bool first = ((iz - z0) & 1) == 0;
+    if (first) {
+      load_round(iz);
+      synth_two<T, NXB, MX>(buf, P, dsyn, a.kxp, tw, tid);
+    }
+    const int base = first ? 0 : 6;
+    T ex[2], ey[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int x = tid + p * NT;
+      const cx<T> q0 = buf[base * padlen(NXB) + pidx(x)];
+      const cx<T> q1 = buf[(base + 1) * padlen(NXB) + pidx(x)];
+      const cx<T> q2 = buf[(base + 2) * padlen(NXB) + pidx(x)];
+      up[p] = q0.x; vp[p] = q0.y;
+      wp[p] = q1.x;
+      if (q1.y < emin) mark_clamp(a.dirty, clampF(iz + 1));
+      ep[p] = q1.y > emin ? q1.y : emin;
+      ex[p] = q2.x; ey[p] = q2.y;
+    }
+    // ---- pointwise tendency (compadv1, complen, compsgs3, sufsim): the
+    // forward pairs (Fx_u,H_u) (Fy_u=Fx_v,H_v) (Fx_w,H_w) (Fy_v,Fy_w)
+    // (Fx_e,H_e) (Fy_e,0) of point p in fa[6p + q], fb[6p + q]
+    T fa[12], fb[12];
+    double sums[7] = {0, 0, 0, 0, 0, 0, 0};
+    const bool top = (iz == nz);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      T xm, x0, xp;
+      if (c.imodel == 4) {                                        // K from k_shear
+        const T* Xr = a.X + (size_t)y * NXB + tid + p * NT;
+        const size_t pp = (size_t)NYB * NXB;
+        xm = Xr[(size_t)(iz - 1) * pp];
+        x0 = Xr[(size_t)iz * pp];
+        xp = top ? x0 : Xr[(size_t)(iz + 1) * pp];             // unused at the top
+      } else {
+        xm = xkk_of(em[p]); x0 = xkk_of(e0[p]); xp = xkk_of(ep[p]);
+      }
+      // the SGS momentum stresses come from the small-grid closure except
+      // at the walls (sufsim_upper at the top face, sufsim below level 1)
+      T tuw0 = 0, tvw0 = 0, tuwm = 0, tvwm = 0;
+      if (top) wall_tau(a.ws, 1, u0[p], v0[p], tuw0, tvw0);
+      if (iz == 1) wall_tau(a.ws, 0, u0[p], v0[p], tuwm, tvwm);
+      T* fap = fa + 6 * p;
+      T* fbp = fb + 6 * p;
+      // compadv1.f90:19-51 and compsgs3.f90:40-59
+      fap[0] = -(u0[p] * u0[p]);
+      // the imposed gradient 0.016 (comp1.f90:152) is a (0,0)-mode constant: k_press adds it in FP64
+      fbp[0] = -((up[p] + u0[p]) * w0[p] - (um[p] + u0[p]) * wm[p]) * todzm - (tuw0 - tuwm) * dzm;
+      fap[1] = -(u0[p] * v0[p]);
+      fbp[1] = -((vp[p] + v0[p]) * w0[p] - (vm[p] + v0[p]) * wm[p]) * todzm - (tvw0 - tvwm) * dzm;
+      fap[3] = -(v0[p] * v0[p]);
+      T Fxw = 0, Fyw = 0, Hw = 0, Fxe = 0, Fye = 0, He = 0;
+      if (!top) {
+        Fxw = -(T(0.5) * (u0[p] + up[p]) * w0[p]);
+        Fyw = -(T(0.5) * (v0[p] + vp[p]) * w0[p]);
+        const T wa = T(0.5) * (wp[p] + w0[p]), wb = T(0.5) * (w0[p] + wm[p]);
+        Hw = -(wa * wa - wb * wb) * dzm;
+        // SGS TKE: advection (compadv1.f90:59-64), diffusion and dissipation
+        // (compsgs3.f90:20-35)
+        Fxe = -T(0.5) * e0[p] * (u0[p] + up[p]) + T(2) * x0 * ex[p];
+        Fye = -T(0.5) * e0[p] * (v0[p] + vp[p]) + T(2) * x0 * ey[p];
+        He = -(wp[p] * ep[p] - wm[p] * em[p]) * todzm +
+             ((xp + x0) * (ep[p] - e0[p]) - (x0 + xm) * (e0[p] - em[p])) * dzm2 - ceps * e0[p] * sqrt(e0[p]);
+      }
+      fap[2] = Fxw; fbp[2] = Hw;
+      fbp[3] = Fyw;
+      fap[4] = Fxe; fbp[4] = He;
+      fap[5] = Fye; fbp[5] = T(0);
+      // SGS stress means uusb .. vwsb: here only the top wall's
+      sums[5] += (double)tuw0;
+      sums[6] += (double)tvw0;
+    }
+    const int rpar = (iz - z0) & 1 ? RN : 0;
+    {
+      const int nq = mn ? 7 : 0;
+      for (int q = 1; q < nq; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sums[q] += __shfl_xor(sums[q], off, 64);
+      }
+      if ((tid & 63) == 0)
+        for (int q = 1; q < nq; ++q) red[rpar + q * NW + (tid >> 6)] = sums[q];
+    }
+    {
+      cx<T>* A = a.A;
+      const size_t o = (size_t)iz * plane + rowoff;
+      const size_t as = a.astride;
+      const T* kxp = a.kxp;
+      auto emit = [&](int q, int k, cx<T> fA, cx<T> fB) {
+        const cx<T> ikx = {0, kxp[k]};
+        switch (q) {
+          case 0: A[0 * as + o + k] = cadd(cmul(ikx, fA), fB); break;          // Gu
+          case 1: A[1 * as + o + k] = fA;                                       // Fyu
+                  A[2 * as + o + k] = cadd(cmul(ikx, fA), fB); break;          // Gv
+          case 2: if (!top) A[4 * as + o + k] = cadd(cmul(ikx, fA), fB); break; // Gw
+          case 3: A[3 * as + o + k] = fA;                                       // Fyv
+                  if (!top) A[5 * as + o + k] = fB; break;                     // Fyw
+          case 4: if (!top) A[6 * as + o + k] = cadd(cmul(ikx, fA), fB); break; // Ge
+          default: if (!top) A[7 * as + o + k] = fA; break;                    // Fye
+        }
+      };
+      anal_pairs<T, NX, 6, true, 2>(buf, fa, fb, tw, tid, emit);
+    }
+    {
+      // wave partials were published by the barriers inside anal_pairs
+      const int nq = mn ? 7 : 0;
+      if (tid >= 1 && tid < nq) {
+        double sq = 0;
+        for (int w = 0; w < NW; ++w) sq += red[rpar + tid * NW + w];
+        a.part[(size_t)tid * a.pstride + (size_t)iz * NYB + y] = sq;
+      }
+    }
+    // slide the window
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      um[p] = u0[p]; u0[p] = up[p];
+      vm[p] = v0[p]; v0[p] = vp[p];
+      wm[p] = w0[p]; w0[p] = wp[p];
+      em[p] = e0[p]; e0[p] = ep[p];
+    }
   }
 }
 

@@ -447,9 +447,12 @@ struct EngineT : Engine {
       }
       const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
       if (!learned) {
-        if (small)
-          hipLaunchKernelGGL((cf::k_rows<T, NX, true>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
-        else
+        if (small) {
+          if constexpr (sizeof(T) == 4 && NX == 512)   // two points per thread, wave-owned transforms
+            hipLaunchKernelGGL((cf::k_rows_lean2<T, NX>), dim3(NYB, nch), dim3(RS::NT / 2), 0, s, a);
+          else
+            hipLaunchKernelGGL((cf::k_rows<T, NX, true>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+        } else
           hipLaunchKernelGGL((cf::k_rows<T, NX, false>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
       }
       // the comp1 clamp of the expanded e, where it fired (no work otherwise)
