@@ -260,13 +260,13 @@ def test_small_grid_closure_matches_oracle(kind, n, nz, stages):
     net.close()
 
 
-@pytest.mark.parametrize("n,nz", [(256, 4), (256, 11), (512, 4)])
+@pytest.mark.parametrize("n,nz", [(256, 4), (256, 11), (512, 4), (512, 11)])
 def test_fp32_small_grid_stage_close_to_fp64(n, nz):
     """The FP32 small-grid closure stage (lean 3/2-grid sweep: wave-owned
-    transforms at 256^2, workgroup-shared at 512^2) against the FP64 one,
-    which the oracle pins, from the same state.  At 256^2 the FP32 sweep
-    synthesises two levels per round; nnz = 11 gives chunks of 8 and 3
-    levels, so the last round of the second chunk holds one level.  With a
+    transforms, at 512^2 on six-wavefront rows with two points per thread)
+    against the FP64 one, which the oracle pins, from the same state.  The
+    FP32 sweep synthesises two levels per round; nnz = 11 gives chunks of 8
+    and 3 levels, so the last round of the second chunk holds one level.  With a
     network that every cell rejects no acceptance decision can flip, so the
     fields differ by FP32 rounding alone: within 2e-5 relative L2, as the
     FP32 TKE step."""
