@@ -1331,11 +1331,10 @@ template <typename T, int NX, bool LEAN = false>
 __global__ void __launch_bounds__(3 * NX / 2) __attribute__((amdgpu_waves_per_eu(rows_prefetch<T, NX>() ? 4 : 1)))
 k_rows(RowArgs<T> a) {
   using F = RowFFT<T, NX>;
-  // Wave-owned transforms, except for the FP32 lean sweep at nnx >= 512: there
-  // the twelve wavefronts of a row share every transform (three synthesis
-  // pairs would leave three quarters of them idle; measured 7.3 -> 6.1 ms per
-  // stage at 512^2 x 256).  The other nnx = 512 kernels spill more that way.
-  constexpr bool WV = F::WAVE && (NX <= 256 || !(LEAN && sizeof(T) == 4));
+  // Wave-owned transforms where rows are whole wavefronts (the FP32 lean
+  // sweep at nnx = 512 is k_rows_lean2, six wavefronts of two points each:
+  // here three synthesis pairs would leave three quarters of twelve idle).
+  constexpr bool WV = F::WAVE;
   constexpr int MX = F::MX, NXB = F::NXB, NT = F::NT;
   constexpr int NYB = 3 * NX / 2;  // square grids: nny == nnx
   // The lean prefetching sweep synthesises two levels per round (synth_two):
