@@ -445,23 +445,23 @@ struct EngineT : Engine {
           learned = true;
         }
       }
-      const int nch = (nz + ctx->zchunk - 1) / ctx->zchunk;
+      // 64-level chunks for the nnx = 512 sweeps from nnz = 128: each chunk
+      // first synthesises its window's pairs, 5 % of a 16-level chunk's
+      // transforms (512^2 x 256: the lean sweep 5.48 -> 5.33 ms per launch,
+      // learned step 52.8 -> 51.7 ms; FP32 TKE 32.2 -> 31.7 ms, FP64 67.4 ->
+      // 66.1 ms).  At nnx 256 16 levels stay (32 measured slower).
+      const int zc = (NX == 512 && nz >= 128) ? 64 : ctx->zchunk;
+      a.c.zchunk = zc;
+      const int nch = (nz + zc - 1) / zc;
       if (!learned) {
         if (small) {
-          if constexpr (sizeof(T) == 4 && NX == 512) {  // two points per thread, wave-owned transforms
-            // 64-level chunks from nnz = 128: each chunk first synthesises
-            // its window's four pairs, 5 % of a 16-level chunk's transforms
-            // (512^2 x 256: 5.48 -> 5.33 ms per launch, learned step 52.8 ->
-            // 51.7 ms; 32 levels 52.3 ms)
-            const int zc = nz >= 128 ? 64 : ctx->zchunk;
-            cf::RowArgs<T> a2 = a;
-            a2.c.zchunk = zc;
-            hipLaunchKernelGGL((cf::k_rows_lean2<T, NX>), dim3(NYB, (nz + zc - 1) / zc), dim3(RS::NT / 2), 0, s,
-                               a2);
-          } else
+          if constexpr (sizeof(T) == 4 && NX == 512)   // two points per thread, wave-owned transforms
+            hipLaunchKernelGGL((cf::k_rows_lean2<T, NX>), dim3(NYB, nch), dim3(RS::NT / 2), 0, s, a);
+          else
             hipLaunchKernelGGL((cf::k_rows<T, NX, true>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
-        } else
+        } else {
           hipLaunchKernelGGL((cf::k_rows<T, NX, false>), dim3(NYB, nch), dim3(RS::NT), 0, s, a);
+        }
       }
       // the comp1 clamp of the expanded e, where it fired (no work otherwise)
       cf::EclampArgs<T> ea{};
