@@ -58,6 +58,12 @@ struct ColShape {
   // k_col_synth / k_col_anal: at least one transform per wavefront of the
   // 512-thread workgroup (they pass one batch of TKC transforms through LDS)
   static constexpr int TKC = TK >= 8 ? TK : (MX < 8 ? MX : 8);
+  // k_sgs_cols_in / k_sgs_cols_out (a batch of 2 TKS transforms): FP32 tiles
+  // of at least 8 columns, 64 bytes of each row, though at ny = 512 the
+  // 75 KB batch leaves two workgroups per CU instead of four (512^2 x 256:
+  // k_sgs_cols_out 1.16 -> 0.84 ms, k_sgs_cols_in 0.87 -> 0.84 ms; learned
+  // step 51.8 -> 50.7 ms, 50.9 ms with k_sgs_cols_in left at 4 columns)
+  static constexpr int TKS = (sizeof(T) == 4 && TK < 8 && MX >= 8) ? 8 : TK;
   // Column stages from registers: with 64 threads per column of a TKC tile
   // and NY = 256, thread (c, j0) = (tid % TKC, tid / TKC) holds band rows
   // j0 + 64 q, q < 4.  On the 3/2 grid (384 = 64 x 6) those are stride-64
@@ -2692,7 +2698,7 @@ struct SgsColsInArgs {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_sgs_cols_in(SgsColsInArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = colpad(NY);
+  constexpr int MX = Sh::MX, TK = Sh::TKS, NT = 512, PL = colpad(NY);
   const int f = blockIdx.z, z = blockIdx.y + 1;
   if (f == 2 && z >= a.nz) return;                 // w lives on faces 1..nz-1
   const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK, tid = threadIdx.x;
@@ -2761,7 +2767,7 @@ struct SgsColsOutArgs {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__(512) k_sgs_cols_out(SgsColsOutArgs<T> a) {
   using Sh = ColShape<T, NX, NY>;
-  constexpr int MX = Sh::MX, TK = Sh::TK, NT = 512, PL = colpad(NY);
+  constexpr int MX = Sh::MX, TK = Sh::TKS, NT = 512, PL = colpad(NY);
   const int cmp = blockIdx.z, z = blockIdx.y + 1;
   if (cmp == 2 && z >= a.nz) return;               // rw(nnz) = 0
   const int k0 = xcd_tile(blockIdx.x, gridDim.x) * TK, tid = threadIdx.x;

@@ -673,7 +673,7 @@ struct EngineT : Engine {
   // row kernel: network, TKE fallback, divergence rows) and sgs_small_out
   // (the y-forward of the divergence rows, added to the comp1 tendencies).
   int sgs_cols_in(cf_ctx* ctx, hipStream_t s) {
-    constexpr int TK = CS::TK;
+    constexpr int TK = CS::TKS;
     cf::SgsColsInArgs<T> a{};
     a.U[0] = cptr(ctx->U);
     a.U[1] = cptr(ctx->V);
@@ -716,7 +716,7 @@ struct EngineT : Engine {
   // tendencies (after k_col_anal has written them).
   int sgs_small_out(cf_ctx* ctx, hipStream_t s) {
     const int nz = ctx->nz;
-    constexpr int TK = CS::TK;
+    constexpr int TK = CS::TKS;
     cf::SgsColsOutArgs<T> b{};
     b.O = cptr(ctx->sgsO);
     b.ostride = (size_t)ctx->slots * NY * MX;
